@@ -1,0 +1,491 @@
+// chromo_device.hpp -- device building blocks of the MI355X Monte Carlo inner loop.
+//
+// Execution model: one wavefront (64 lanes, one workgroup) owns one replica's
+// Markov chain and executes its moves one after another.  Scalar parts of a move
+// (random draws, bead selection, the rotation matrix, the Metropolis test) run as
+// wave-uniform code; per-bead and per-voxel parts spread over the lanes.
+//
+// All floating-point code is compiled with -ffp-contract=off and keeps the
+// evaluation order of the reference (chromo/*.pyx), so that positions, voxel
+// indices, delta densities and decisions reproduce the reference arithmetic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "chromo_hip.h"
+
+namespace chromo {
+
+constexpr int WAVE = 64;
+constexpr int SEG_LDS = 32;                 // moves with n_inds <= SEG_LDS use the LDS arena
+constexpr int HASH_LDS = 1024;              // >= 2 * 16 * SEG_LDS
+constexpr int MAX_CTRL = 8;
+constexpr double RMAX = 2147483647.0;       // RAND_MAX as a double
+constexpr double E_HUGE_FIELD = 1E99;       // chromo/fields.pyx:32
+constexpr double E_HUGE_POLY = 1E25;        // chromo/polymers.pyx:35
+constexpr int HKEY_EMPTY = -1;
+constexpr int HMIN_EMPTY = 0x7fffffff;
+
+enum : int {
+    ERR_WINDOW = 1,       // bead selection window larger than the chain (ValueError)
+    ERR_SEGCAP = 2,       // move larger than the scratch arena
+    ERR_TANGENT_K = 3,    // more distinct beads requested than the chain has
+    ERR_NBINDERS = 4,     // binding move on a chain without binders
+};
+
+__device__ __forceinline__ int lane_id() { return (int)threadIdx.x; }
+
+// ---------------------------------------------------------------------------
+// Wave-level helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int l)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+
+__device__ __forceinline__ double bcast0(double x) { return __shfl(x, 0, WAVE); }
+__device__ __forceinline__ int64_t bcast0(int64_t x) { return __shfl(x, 0, WAVE); }
+__device__ __forceinline__ int bcast0(int x) { return __shfl(x, 0, WAVE); }
+
+// Fixed-order butterfly sum; every lane returns lane 0's result.
+__device__ __forceinline__ double wave_sum(double x)
+{
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, WAVE);
+    return bcast0(x);
+}
+
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t x)
+{
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, WAVE);
+    return x;
+}
+
+__device__ __forceinline__ int wave_sum_int(int x)
+{
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, WAVE);
+    return x;
+}
+
+// Exclusive prefix sum over the wave; total receives the wave sum.
+__device__ __forceinline__ int wave_excl_scan(int x, int &total)
+{
+    int v = x;
+    const int lane = lane_id();
+    for (int d = 1; d < WAVE; d <<= 1) {
+        int y = __shfl_up(v, d, WAVE);
+        if (lane >= d) v += y;
+    }
+    total = __shfl(v, WAVE - 1, WAVE);
+    return v - x;
+}
+
+// ---------------------------------------------------------------------------
+// Random streams (wave-uniform)
+// ---------------------------------------------------------------------------
+
+// glibc random_r TYPE_3 state: lane j (j < 31) holds word j; the front and rear
+// indices are wave-uniform.  Replaces libc rand() as called throughout the
+// reference (chromo/mc/mc_sim.pyx:171, chromo/mc/move_funcs.pyx,
+// chromo/util/linalg.pyx, chromo/util/bead_selection.pyx).
+struct WaveRng {
+    int mode;
+    uint32_t g;
+    int gf, gr;
+    uint32_t *mt;       // 624 words in LDS (numpy MT19937 state)
+    int mti;
+    uint32_t k0, k1;
+    uint64_t ctr;
+    uint32_t b0, b1, b2, b3;
+    int bi;
+};
+
+__device__ __forceinline__ int32_t glibc_draw(WaveRng &R)
+{
+    uint32_t a = readlane_u32(R.g, R.gf);
+    uint32_t b = readlane_u32(R.g, R.gr);
+    uint32_t v = a + b;
+    if (lane_id() == R.gf) R.g = v;
+    if (++R.gf >= 31) {
+        R.gf = 0;
+        ++R.gr;
+    } else if (++R.gr >= 31) {
+        R.gr = 0;
+    }
+    return (int32_t)(v >> 1);
+}
+
+// Philox4x32-10 (Salmon et al., SC'11).
+__device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+        uint32_t n0 = hi1 ^ c[1] ^ k0;
+        uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+    }
+}
+
+__device__ __forceinline__ uint32_t philox_draw(WaveRng &R)
+{
+    if (R.bi >= 4) {
+        uint32_t c[4] = {(uint32_t)R.ctr, (uint32_t)(R.ctr >> 32), 0u, 0u};
+        philox10(c, R.k0, R.k1);
+        R.b0 = c[0];
+        R.b1 = c[1];
+        R.b2 = c[2];
+        R.b3 = c[3];
+        R.ctr++;
+        R.bi = 0;
+    }
+    uint32_t v = R.bi == 0 ? R.b0 : (R.bi == 1 ? R.b1 : (R.bi == 2 ? R.b2 : R.b3));
+    R.bi++;
+    return v;
+}
+
+// MT19937 regeneration of the 624-word block (lane 0, LDS), then tempering.
+__device__ void mt_twist(uint32_t *mt)
+{
+    __syncthreads();
+    if (lane_id() == 0) {
+        for (int i = 0; i < 624; ++i) {
+            uint32_t y = (mt[i] & 0x80000000u) | (mt[i + 1 == 624 ? 0 : i + 1] & 0x7fffffffu);
+            int j = i + 397;
+            if (j >= 624) j -= 624;
+            uint32_t v = mt[j] ^ (y >> 1);
+            if (y & 1u) v ^= 0x9908b0dfu;
+            mt[i] = v;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t mt_draw(WaveRng &R)
+{
+    if (R.mti >= 624) {
+        mt_twist(R.mt);
+        R.mti = 0;
+    }
+    uint32_t y = R.mt[R.mti++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+__device__ __forceinline__ int32_t rnd31(WaveRng &R)
+{
+    if (R.mode == CHROMO_RNG_GLIBC) return glibc_draw(R);
+    return (int32_t)(philox_draw(R) >> 1);
+}
+
+__device__ __forceinline__ double rnd_unit(WaveRng &R) { return (double)rnd31(R) / RMAX; }
+
+// numpy legacy RandomState.randint(0, high): masked rejection over 32-bit draws
+// (chromo/mc/move_funcs.pyx:819).
+__device__ __forceinline__ int64_t rnd_randint(WaveRng &R, int64_t high)
+{
+    uint32_t rng = (uint32_t)(high - 1);
+    if (rng == 0) return 0;
+    uint32_t mask = rng;
+    mask |= mask >> 1;
+    mask |= mask >> 2;
+    mask |= mask >> 4;
+    mask |= mask >> 8;
+    mask |= mask >> 16;
+    for (;;) {
+        uint32_t v = (R.mode == CHROMO_RNG_GLIBC ? mt_draw(R) : philox_draw(R)) & mask;
+        if (v <= rng) return (int64_t)v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Geometry (chromo/util/linalg.pyx, chromo/mc/move_funcs.pyx)
+// ---------------------------------------------------------------------------
+
+// uniform_sample_unit_sphere, linalg.pyx:23-59
+__device__ __forceinline__ void sample_unit_sphere(WaveRng &R, double v[3])
+{
+    double phi = (double)rnd31(R) / RMAX * (2.0 * M_PI);
+    double theta = acos(((double)rnd31(R) / RMAX) * 2 - 1);
+    v[0] = cos(phi) * sin(theta);
+    v[1] = sin(phi) * sin(theta);
+    v[2] = cos(theta);
+}
+
+// arbitrary_axis_rotation, linalg.pyx:62-139 (rows 0..2 of the 4x4 matrix)
+__device__ __forceinline__ void axis_rotation(const double a[3], const double p[3], double ang,
+                                              double T[12])
+{
+    double c = cos(ang), s = sin(ang);
+    T[0] = a[0] * a[0] + (a[1] * a[1] + a[2] * a[2]) * c;
+    T[1] = a[0] * a[1] * (1 - c) - a[2] * s;
+    T[2] = a[0] * a[2] * (1 - c) + a[1] * s;
+    T[4] = a[0] * a[1] * (1 - c) + a[2] * s;
+    T[5] = a[1] * a[1] + (a[0] * a[0] + a[2] * a[2]) * c;
+    T[6] = a[1] * a[2] * (1 - c) - a[0] * s;
+    T[8] = a[0] * a[2] * (1 - c) - a[1] * s;
+    T[9] = a[1] * a[2] * (1 - c) + a[0] * s;
+    T[10] = a[2] * a[2] + (a[0] * a[0] + a[1] * a[1]) * c;
+    double rv0 = (p[1] * a[2] - p[2] * a[1]) * s;
+    double rv1 = (p[2] * a[0] - p[0] * a[2]) * s;
+    double rv2 = (p[0] * a[1] - p[1] * a[0]) * s;
+    rv0 += (p[0] * (1 - a[0] * a[0]) - a[0] * (p[1] * a[1] + p[2] * a[2])) * (1 - c);
+    rv1 += (p[1] * (1 - a[1] * a[1]) - a[1] * (p[0] * a[0] + p[2] * a[2])) * (1 - c);
+    rv2 += (p[2] * (1 - a[2] * a[2]) - a[2] * (p[0] * a[0] + p[1] * a[1])) * (1 - c);
+    T[3] = rv0;
+    T[7] = rv1;
+    T[11] = rv2;
+}
+
+// One row of transform_r_t3_t2 (move_funcs.pyx:139-154): left fold from zero.
+__device__ __forceinline__ double rowdot(const double *Trow, double x0, double x1, double x2)
+{
+    double e = 0;
+    e += Trow[0] * x0;
+    e += Trow[1] * x1;
+    e += Trow[2] * x2;
+    return e;
+}
+
+// Cython float modulo with cdivision=False (fmod + sign fix).
+__device__ __forceinline__ double pymod(double a, double b)
+{
+    double r = fmod(a, b);
+    r += (double)((r != 0) & ((r < 0) ^ (b < 0))) * b;
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Bead selection (chromo/util/bead_selection.pyx), wave-uniform
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int64_t imax64(int64_t a, int64_t b) { return a > b ? a : b; }
+__device__ __forceinline__ int64_t imin64(int64_t a, int64_t b) { return a < b ? a : b; }
+
+// capped_exponential, bead_selection.pyx:19-66
+__device__ int64_t capped_exponential(WaveRng &R, int64_t window, int64_t cap)
+{
+    int64_t r;
+    do {
+        r = (int64_t)(-log10(((double)rnd31(R) / RMAX) + 0.00001) * (double)window * 0.45 +
+                      1.0001);
+    } while (r > cap);
+    return r;
+}
+
+__device__ __forceinline__ int64_t from_left(WaveRng &R, int64_t w, int64_t nb, int &err)
+{
+    if (w > nb) {
+        err = ERR_WINDOW;
+        return 0;
+    }
+    return capped_exponential(R, w, w);
+}
+
+__device__ __forceinline__ int64_t from_right(WaveRng &R, int64_t w, int64_t nb, int &err)
+{
+    int64_t d = from_left(R, w, nb, err);
+    return nb - d;
+}
+
+__device__ int64_t from_point(WaveRng &R, int64_t w, int64_t nb, int64_t ind0, int &err)
+{
+    if (w > nb) {
+        err = ERR_WINDOW;
+        return 0;
+    }
+    if (w < 1) return ind0;
+    int64_t side = rnd31(R) % 2;
+    if (side == 0) {
+        int64_t ws = imax64(imin64(w, ind0), 1);
+        int64_t ub = imax64(ind0, 1);
+        return from_right(R, ws, ub, err);
+    }
+    int64_t ws = imax64(imin64(w, nb - ind0), 1);
+    int64_t ub = imax64(nb - ind0, 1);
+    return from_left(R, ws, ub, err) + ind0;
+}
+
+__device__ __forceinline__ void check_bead_bounds(int64_t b0, int64_t b1, int64_t nb, int64_t &ind0,
+                                                  int64_t &indf)
+{
+    b0 = imin64(b0, nb);
+    b0 = imax64(b0, 0);
+    if (b1 > nb) {
+        ind0 = b0;
+        indf = nb;
+    } else if (b1 < 0) {
+        ind0 = 0;
+        indf = b0 + 1;
+    } else if (b0 == b1) {
+        ind0 = b0;
+        indf = b0 + 1;
+    } else {
+        ind0 = imin64(b0, b1);
+        indf = imax64(b0, b1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SSWLC elastic energy (chromo/polymers.pyx:1148-1346)
+// ---------------------------------------------------------------------------
+
+struct Bond {
+    double eb, ep, eperp, gam, eta;
+};
+
+__device__ __forceinline__ Bond load_bond(const double *bonds, int64_t N, int64_t b)
+{
+    const int64_t M = N - 1;
+    Bond o;
+    o.eb = bonds[b];
+    o.ep = bonds[M + b];
+    o.eperp = bonds[2 * M + b];
+    o.gam = bonds[3 * M + b];
+    o.eta = bonds[4 * M + b];
+    return o;
+}
+
+__device__ __forceinline__ double dot3(const double *a, const double *b)
+{
+    double d = 0;
+    d += a[0] * b[0];
+    d += a[1] * b[1];
+    d += a[2] * b[2];
+    return d;
+}
+
+// E_pair, polymers.pyx:1148-1175 ((x)**2 is pow(x, 2.0) == x*x under glibc)
+__device__ __forceinline__ double E_pair(const Bond &bd, const double *bend, double par,
+                                         const double *perp)
+{
+    double q = par - bd.gam;
+    return 0.5 * bd.eb * dot3(bend, bend) + 0.5 * bd.ep * (q * q) + 0.5 * bd.eperp * dot3(perp, perp);
+}
+
+// bead_pair_dE_poly_forward, polymers.pyx:1177-1277
+__device__ double pair_dE_forward(const Bond &bd, const double *r0, const double *r1,
+                                  const double *tr1, const double *t30, const double *t31,
+                                  const double *tt31)
+{
+    double dr_t[3], dr[3], drp_t[3], drp[3], be_t[3], be[3];
+    for (int i = 0; i < 3; ++i) {
+        dr_t[i] = tr1[i] - r0[i];
+        dr[i] = r1[i] - r0[i];
+    }
+    double par_t = dot3(t30, dr_t), par = dot3(t30, dr);
+    for (int i = 0; i < 3; ++i) {
+        drp_t[i] = dr_t[i] - t30[i] * par_t;
+        drp[i] = dr[i] - t30[i] * par;
+        be_t[i] = tt31[i] - t30[i] - drp_t[i] * bd.eta;
+        be[i] = t31[i] - t30[i] - drp[i] * bd.eta;
+    }
+    return E_pair(bd, be_t, par_t, drp_t) - E_pair(bd, be, par, drp);
+}
+
+// bead_pair_dE_poly_reverse, polymers.pyx:1279-1346
+__device__ double pair_dE_reverse(const Bond &bd, const double *r0, const double *tr0,
+                                  const double *r1, const double *t30, const double *tt30,
+                                  const double *t31)
+{
+    double dr_t[3], dr[3], drp_t[3], drp[3], be_t[3], be[3];
+    for (int i = 0; i < 3; ++i) {
+        dr_t[i] = r1[i] - tr0[i];
+        dr[i] = r1[i] - r0[i];
+    }
+    double par_t = dot3(tt30, dr_t), par = dot3(t30, dr);
+    for (int i = 0; i < 3; ++i) {
+        drp_t[i] = dr_t[i] - tt30[i] * par_t;
+        drp[i] = dr[i] - t30[i] * par;
+        be_t[i] = t31[i] - tt30[i] - drp_t[i] * bd.eta;
+        be[i] = t31[i] - t30[i] - drp[i] * bd.eta;
+    }
+    return E_pair(bd, be_t, par_t, drp_t) - E_pair(bd, be, par, drp);
+}
+
+// bead_binding_dE, polymers.pyx:1408-1538, with host-built -log(single-site Z)
+// tables F[b][marks][state].
+template <int NB>
+__device__ double bead_binding_dE(const chromo_replica *R, const int8_t *mods, const int8_t *st,
+                                  const int8_t *stt)
+{
+    constexpr int B = NB;
+    double dE = 0;
+    if (R->max_binders != -1) {
+        int64_t tot = 0;
+        for (int h = 0; h < B; ++h) tot += stt[h];
+        if (tot > R->max_binders) dE += E_HUGE_POLY * (double)(tot - R->max_binders);
+        tot = 0;
+        for (int h = 0; h < B; ++h) tot += st[h];
+        if (tot > R->max_binders) dE -= E_HUGE_POLY * (double)(tot - R->max_binders);
+    }
+    const double f = R->mu_adjust;
+    for (int b = 0; b < B; ++b) {
+        dE += R->F[b][mods[b]][stt[b]];
+        dE -= R->F[b][mods[b]][st[b]];
+        const double mu = R->mu[b];
+        if (mu > 0) {
+            dE -= (double)stt[b] * (mu * (-f + 2));
+            dE += (double)st[b] * (mu * (-f + 2));
+        } else {
+            dE -= (double)stt[b] * mu * f;
+            dE += (double)st[b] * mu * f;
+        }
+    }
+    return dE;
+}
+
+// ---------------------------------------------------------------------------
+// Field geometry (chromo/fields.pyx:1434-1673)
+// ---------------------------------------------------------------------------
+
+// Lower voxel indices and lower-corner weights of one position.
+__device__ __forceinline__ void voxel_lower(const chromo_replica *R, const double *pos, int64_t ix[3],
+                                            double w[3])
+{
+    const int64_t nn[3] = {R->nx, R->ny, R->nz};
+    for (int j = 0; j < 3; ++j) {
+        double x = pymod(pos[j] + R->half_width[j], R->width[j]) - R->half_step[j];
+        double q = x / R->dxyz[j];
+        // finite coordinates give q in [-0.5, n - 0.5); a non-finite one is pinned to
+        // voxel 0 so that no index can leave the grid
+        if (!(q >= -1.0 && q < (double)nn[j])) q = 0.0;
+        int64_t ind = (int64_t)floor(q);
+        ix[j] = (ind == -1) ? nn[j] - 1 : ind;
+        w[j] = 1 - (q - (double)ind);
+    }
+}
+
+// Voxel super-index of corner l (x fastest) with the wrap of
+// precompute_ind_xyz_to_super (fields.pyx:673-685).
+__device__ __forceinline__ int64_t corner_voxel(const chromo_replica *R, const int64_t ix[3], int l)
+{
+    int64_t x = ix[0] + (l & 1), y = ix[1] + ((l >> 1) & 1), z = ix[2] + ((l >> 2) & 1);
+    if (x >= R->nx) x -= R->nx;
+    if (y >= R->ny) y -= R->ny;
+    if (z >= R->nz) z -= R->nz;
+    return x + y * R->nx + z * R->nx * R->ny;
+}
+
+// Corner weight products of _generate_weight_vector_with_trial (fields.pyx:1524-1598).
+__device__ __forceinline__ double corner_weight(int l, double wx, double wy, double wz)
+{
+    double a = (l & 1) ? (1 - wx) : wx;
+    double b = (l & 2) ? (1 - wy) : wy;
+    double c = (l & 4) ? (1 - wz) : wz;
+    return a * b * c;
+}
+
+}  // namespace chromo

@@ -1,0 +1,188 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bar: voxel indices, unique-voxel sets, binder states, proposal bead ranges and
+accept/reject decisions identical; per-voxel delta densities and the initial
+density grid bit-identical; dE within 1e-9 kT (absolute, and relative for
+E_HUGE penalties); positions within 1e-9 nm (device sin/cos/acos/log10 may
+differ from glibc's by an ulp).
+"""
+import copy
+import sys
+
+import numpy as np
+import pytest
+
+from chromo_amd import _lib as L
+from chromo_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _dev(systems, **kw):
+    from chromo_amd.engine import DeviceReplicas
+    return DeviceReplicas([s.chain for s in systems], [s.field for s in systems],
+                          [copy.deepcopy(s.ctrls) for s in systems], **kw)
+
+
+def _dE_close(a, b):
+    tol = 1e-9 + 1e-12 * np.maximum(np.abs(a), np.abs(b))
+    return np.abs(a - b) <= tol
+
+
+# ---------------------------------------------------------------- density ---
+
+@pytest.mark.parametrize("N,nvox,seed", [(300, 7, 0), (2000, 16, 1), (5000, 12, 2), (64, 1, 3),
+                                         (64, 2, 4), (1000, 5, 5)])
+def test_density_init_bitwise(oracle, N, nvox, seed):
+    say("density_init", N, nvox, seed)
+    s = W.make_system(N, 2, nvox, seed=seed)
+    s.chain.states = np.random.RandomState(seed).randint(0, 3, size=(N, 2))
+    dr = _dev([s])
+    g = dr.density_of(0)
+    o = oracle.OracleSim(s.chain, s.field).density()
+    assert np.array_equal(g, o)
+
+
+def test_density_init_edge_positions(oracle):
+    say("density_init edges")
+    s = W.make_system(8, 2, 4, seed=0)
+    Wd = s.field.x_width
+    dx = Wd / 4
+    s.chain.r = np.array([[-Wd / 2, 0, 0], [Wd / 2, Wd / 2, Wd / 2], [-Wd / 2 + 0.5 * dx] * 3,
+                          [0, 0, 0], [Wd / 2 - 1e-12, -Wd / 2 + 1e-12, 3 * Wd], [dx, -dx, 2 * dx],
+                          [-7 * Wd, 5.5 * Wd, 0.25], [1e-300, -1e-300, 0.0]])
+    s.chain.states = np.array([[1, 2]] * 8)
+    dr = _dev([s])
+    assert np.array_equal(dr.density_of(0), oracle.OracleSim(s.chain, s.field).density())
+
+
+# ------------------------------------------------------------------- tape ---
+
+@pytest.mark.parametrize("n_move", [1, 3, 17, 32, 33, 80, 150])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_field_tape(oracle, n_move, seed):
+    say("field_tape", n_move, seed)
+    N = 400
+    s = W.make_system(N, 2, 9, seed=seed, eps=[-3.0, -2.0], cross_eps=0.7, chi=1.4)
+    rs = np.random.RandomState(seed)
+    s.chain.states = rs.randint(0, 3, size=(N, 2))
+    o = oracle.OracleSim(s.chain, s.field)
+    dr = _dev([s])
+    for trial in range(4):
+        ind0 = rs.randint(0, N - n_move)
+        inds = np.arange(ind0, ind0 + n_move)
+        a = rs.normal(size=3)
+        T = oracle.axis_rotation(a / np.linalg.norm(a), s.chain.r[ind0], rs.uniform(-1.5, 1.5))
+        rt, _, _ = oracle.transform(T, s.chain.r[inds], s.chain.t3[inds], s.chain.t2[inds])
+        st = s.chain.states[inds]
+        dEo, bins_o, delta_o = o.field_dE_tape(inds, rt, st, 0)
+        dEg, bins_g, delta_g = dr.field_dE_tape(0, inds, rt, st.astype(np.int8), False)
+        assert sorted(bins_g) == sorted(bins_o)
+        assert np.array_equal(delta_g, delta_o[bins_g])           # per-voxel fold, bit for bit
+        assert _dE_close(dEg, dEo)
+        # binding (state change) on one bead
+        k = rs.randint(0, N)
+        st2 = s.chain.states[[k]].copy()
+        st2[0, trial % 2] = (st2[0, trial % 2] + 1) % 3
+        dEo, bins_o, delta_o = o.field_dE_tape([k], s.chain.r[[k]], st2, 1)
+        dEg, bins_g, delta_g = dr.field_dE_tape(0, [k], s.chain.r[[k]], st2.astype(np.int8), True)
+        assert sorted(bins_g) == sorted(bins_o)
+        assert np.array_equal(delta_g, delta_o[bins_g])
+        assert _dE_close(dEg, dEo)
+
+
+# ---------------------------------------------------------------- mc_sim ---
+
+def _compare_run(oracle, s, steps, inner_seed=123, rng_mode=0, seed=1, stream=0, launch="run"):
+    ctrls_o = copy.deepcopy(s.ctrls)
+    o = oracle.OracleSim(s.chain, s.field, rng_mode=rng_mode, seed=seed, stream=stream)
+    o.np_seed(inner_seed)
+    lo = o.mc_sim(steps, ctrls_o, max_logs=10 ** 6)
+    dr = _dev([s], rng_mode=rng_mode, seeds=[seed], streams=[stream], log_cap=len(lo) + 100)
+    dr.np_seed([inner_seed])
+    if launch == "run":
+        dr.run(steps)
+    else:
+        dr.run_plain(steps)
+    lg = dr.logs(0)
+    assert len(lg) == len(lo)
+    for f in ("move", "n", "ind0", "indf", "n_bins"):
+        assert np.array_equal(lg[f], lo[f]), f
+    assert np.array_equal(lg["accepted"], lo["accepted"])
+    assert np.all(_dE_close(lg["dE_poly"], lo["dE_poly"]))
+    assert np.all(_dE_close(lg["dE_field"], lo["dE_field"]))
+    assert np.array_equal(lg["u"], lo["u"])
+    r, t3, t2 = dr.positions(0)
+    ro, t3o, t2o, sto = o.state()
+    assert np.array_equal(dr.binder_states(0), sto)
+    assert np.max(np.abs(r - ro)) < 1e-9 and np.max(np.abs(t3 - t3o)) < 1e-12
+    assert np.max(np.abs(t2 - t2o)) < 1e-12
+    assert np.allclose(dr.density_of(0), o.density(), rtol=1e-12, atol=1e-20)
+    cg = dr.controllers(0)
+    for a, b in zip(cg, ctrls_o):
+        assert a.amp_bead == b.amp_bead and a.num_attempt == b.num_attempt
+        assert a.num_success == b.num_success and a.rate == b.rate
+        assert abs(a.amp_move - b.amp_move) <= 1e-15 * abs(b.amp_move)
+    cnt = dr.counter_array()[0]
+    assert cnt[L.CNT_PROPOSALS] == len(lo) and cnt[L.CNT_ERROR] == 0
+    return lo
+
+
+@pytest.mark.parametrize("N,nvox,seed", [(300, 8, 11), (1000, 30, 12), (5000, 20, 13)])
+def test_mc_sim_glibc_parity(oracle, N, nvox, seed):
+    say("mc_sim glibc", N, nvox, seed)
+    s = W.make_system(N, 2, nvox, seed=seed)
+    lo = _compare_run(oracle, s, steps=15)
+    assert lo["accepted"].sum() > 100
+
+
+def test_mc_sim_philox_parity(oracle):
+    say("mc_sim philox")
+    s = W.make_system(800, 2, 12, seed=21, chi=0.7, eps=[-4.5, -1.5], mu=[-1.0, 0.5])
+    _compare_run(oracle, s, steps=12, rng_mode=1, seed=777, stream=5)
+
+
+def test_mc_sim_three_binders_and_cross_talk(oracle):
+    say("mc_sim B=3")
+    s = W.make_system(600, 3, 10, seed=22, eps=[-4, -3, -2], cross_eps=0.5, mu=[-0.4, 0.2, -1])
+    _compare_run(oracle, s, steps=10)
+
+
+def test_mc_sim_large_moves_use_global_arena(oracle):
+    """Crank and slide windows up to 150 beads: moves above 32 beads run through
+    the global scratch arena."""
+    say("mc_sim large moves")
+    s = W.make_system(1500, 2, 16, seed=23)
+    for c in s.ctrls:
+        if c.move in (L.CRANK_SHAFT, L.SLIDE, L.END_PIVOT):
+            c.amp_bead = int(c.bead_hi)
+            c.bead_lo = c.bead_hi
+    lo = _compare_run(oracle, s, steps=6)
+    assert lo["n"].max() > 32
+
+
+def test_plain_equals_persistent(oracle):
+    say("plain vs persistent")
+    s = W.make_system(500, 2, 10, seed=24)
+    _compare_run(oracle, s, steps=4, launch="plain")
+
+
+def test_batched_equals_single(oracle):
+    """Replicas advanced together in one launch give the same chains as alone."""
+    say("batched vs single")
+    systems = [W.make_system(n, 2, g, seed=30 + i, chi=0.5 + 0.2 * i)
+               for i, (n, g) in enumerate([(300, 8), (900, 12), (2000, 16), (450, 9)])]
+    together = _dev(systems, rng_mode=1, seeds=[5] * 4, streams=[0, 1, 2, 3])
+    together.run(8)
+    for i, s in enumerate(systems):
+        alone = _dev([s], rng_mode=1, seeds=[5], streams=[i])
+        alone.run(8)
+        for a, b in zip(together.positions(i), alone.positions(0)):
+            assert np.array_equal(a, b)
+        assert np.array_equal(together.binder_states(i), alone.binder_states(0))
+        assert np.array_equal(together.density_of(i), alone.density_of(0))
