@@ -1,0 +1,216 @@
+"""Benchmark: chromo MC move-evaluations per second on one MI355X, batched vs plain.
+
+Headline workload (``value``): 512 independent replicas advanced together by
+the persistent kernel, chain lengths N ~ U{2000..50000}, 32^3..64^3 voxels of
+30 nm, per-replica chi ~ U[0,2], eps ~ U[-5,-1], mu ~ U[-2,1], Bernoulli(0.3)
+marks, Philox streams.  One *step* is one mc_sim sweep of every replica
+(all_moves: 161 proposals per replica) in one kernel launch.
+
+Also reported: the plain path (one kernel launch per proposal) on the
+100,000-bead / 64^3 configuration, the ratio of the two, a dE / decision check
+of four replicas against the CPU oracle, the roofline of the dominant kernel
+(algorithmic HBM bytes per launch over its HIP-event-timed duration), and the
+CPU oracle timed on a bounded sample of the same replicas on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import copy
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "MC move-evals/s on 1 GPU, batched vs plain; dE err <=1e-9 kT; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_replicas(systems, rng_mode, seeds, streams, log_cap=0):
+    from chromo_amd.engine import DeviceReplicas
+    return DeviceReplicas([s.chain for s in systems], [s.field for s in systems],
+                          [copy.deepcopy(s.ctrls) for s in systems], rng_mode=rng_mode,
+                          seeds=seeds, streams=streams, log_cap=log_cap)
+
+
+def timed(fn, n):
+    import torch
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(n):
+        fn()
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / 1000.0
+
+
+def bench_batched(args):
+    from chromo_amd import _lib as L
+    from chromo_amd import workloads as W
+    t0 = time.time()
+    systems = W.config_batched(args.replicas, seed=3)
+    dr = build_replicas(systems, L.RNG_PHILOX, [20240601] * len(systems),
+                        list(range(len(systems))))
+    log(f"batched: {len(systems)} replicas built in {time.time() - t0:.1f}s "
+        f"(beads {sum(s.chain.N for s in systems)})")
+    for _ in range(args.warmup):
+        dr.run(1, sync=False)
+    dr.raise_on_error()
+    c0 = dr.counter_array()
+    secs = timed(lambda: dr.run(1, sync=False), args.steps)
+    dr.raise_on_error()
+    c1 = dr.counter_array()
+    d = c1 - c0
+    props = int(d[:, L.CNT_PROPOSALS].sum())
+    nbytes = int(d[:, L.CNT_BYTES].sum())
+    return dict(secs=secs, proposals=props, bytes=nbytes, accepted=int(d[:, L.CNT_ACCEPTED].sum()),
+                near_ties=int(d[:, L.CNT_NEAR_TIES].sum()), replicas=len(systems),
+                beads=int(sum(s.chain.N for s in systems)), systems=systems)
+
+
+def bench_plain(args):
+    from chromo_amd import _lib as L
+    from chromo_amd import workloads as W
+    s = W.config_plain_gpu(seed=2)
+    dr = build_replicas([s], L.RNG_GLIBC, [1], None)
+    dr.run_plain(1)
+    c0 = dr.counter_array()
+    secs = timed(lambda: dr.run_plain(1, sync=False), args.plain_steps)
+    dr.raise_on_error()
+    c1 = dr.counter_array()
+    d = (c1 - c0)[0]
+    return dict(secs=secs, proposals=int(d[L.CNT_PROPOSALS]), launches=args.plain_steps * (161 + 5),
+                bytes=int(d[L.CNT_BYTES]))
+
+
+def dE_check(systems, sweeps=20, n=4):
+    """GPU vs oracle on the first n replicas of the batched workload."""
+    from chromo_amd import _lib as L
+    from oracle import oracle as O
+    worst, mism, total = 0.0, 0, 0
+    for i in range(n):
+        s = systems[i]
+        o = O.OracleSim(s.chain, s.field, rng_mode=O.RNG_PHILOX, seed=20240601, stream=i)
+        lo = o.mc_sim(sweeps, copy.deepcopy(s.ctrls), max_logs=10 ** 6)
+        dr = build_replicas([s], L.RNG_PHILOX, [20240601], [i], log_cap=len(lo) + 16)
+        dr.run(sweeps)
+        lg = dr.logs(0)
+        if len(lg) != len(lo):
+            mism += abs(len(lg) - len(lo))
+        k = min(len(lg), len(lo))
+        total += k
+        mism += int(np.sum(lg["accepted"][:k] != lo["accepted"][:k]))
+        dg = lg["dE_poly"][:k] + lg["dE_field"][:k]
+        do = lo["dE_poly"][:k] + lo["dE_field"][:k]
+        ok = np.abs(do) < 1e50
+        if ok.any():
+            worst = max(worst, float(np.max(np.abs(dg[ok] - do[ok]))))
+    return dict(proposals=total, max_abs_dE_err=worst, decision_mismatches=mism)
+
+
+def cpu_baseline(systems, target_s=15.0):
+    """The C oracle (lean build) on a bounded sample of the batched replicas."""
+    from oracle import oracle as O
+    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+                         else os.cpu_count() or 1))
+    sample = systems[:threads]
+    sims = [O.OracleSim(s.chain, s.field, rng_mode=O.RNG_PHILOX, seed=20240601, stream=i)
+            for i, s in enumerate(sample)]
+    ctrls = [copy.deepcopy(s.ctrls) for s in sample]
+    # calibrate: one sweep, then enough sweeps for ~target_s
+    p, sec = O.run_batch(sims, ctrls, 1, threads)
+    sweeps = int(max(1, min(2000, target_s / max(sec, 1e-3))))
+    p, sec = O.run_batch(sims, ctrls, sweeps, threads)
+    return dict(value=p / sec, unit="move-evals/s", cores=threads, kind="port",
+                sample=f"{len(sample)} batched replicas (one per thread), {sweeps} sweeps, "
+                       f"{p} proposals in {sec:.1f} s, oracle lean build")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--replicas", type=int, default=512)
+    ap.add_argument("--plain-steps", type=int, default=40)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: replicas are independent chains; multi-GPU runs "
+              "would shard replicas with no exchange and are not part of this build.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "needs a ROCm GPU"
+
+    b = bench_batched(args)
+    value = b["proposals"] / b["secs"]
+    log(f"batched: {b['proposals']} proposals in {b['secs']:.3f}s -> {value:.4g}/s")
+    p = bench_plain(args)
+    plain_value = p["proposals"] / p["secs"]
+    log(f"plain: {p['proposals']} proposals in {p['secs']:.3f}s -> {plain_value:.4g}/s")
+    chk = None if args.no_check else dE_check(b["systems"])
+    if chk:
+        log(f"check: {chk}")
+    cpu = None if args.no_cpu_baseline else cpu_baseline(b["systems"])
+    if cpu:
+        log(f"cpu: {cpu}")
+
+    achieved = b["bytes"] / b["secs"] / 1e9
+    traffic = None
+    prof = ROOT / "profiles" / "traffic_k_mc_run.json"
+    if prof.exists():
+        try:
+            traffic = json.loads(prof.read_text()).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "move-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * b["secs"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched: {b['replicas']} replicas, N~U{{2000..50000}} "
+                        f"({b['beads']} beads), 32^3-64^3 voxels of 30 nm, B=2, all_moves + "
+                        "SimpleControl, Philox; one step = one sweep (161 proposals per replica)",
+            "replicas": b["replicas"],
+            "parallelism": "replicas (one wavefront per replica)",
+        },
+        "plain": {"value": plain_value, "unit": "move-evals/s",
+                  "workload": "100,000 beads, 64^3 voxels, B=2, one kernel launch per proposal",
+                  "proposals": p["proposals"], "seconds": p["secs"]},
+        "batched_over_plain": value / plain_value,
+        "acceptance": b["accepted"] / max(1, b["proposals"]),
+        "near_ties": b["near_ties"],
+        "dE_check": chk,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_mc_run",
+                     "algorithmic_bytes_per_launch": b["bytes"] / args.steps},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
