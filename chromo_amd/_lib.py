@@ -84,6 +84,20 @@ class ChromoReplica(ctypes.Structure):
     ]
 
 
+class ChromoEnergyModel(ctypes.Structure):
+    _fields_ = [
+        ("N", ctypes.c_int64), ("B", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("t3", ctypes.c_void_p), ("states", ctypes.c_void_p), ("marks", ctypes.c_void_p),
+        ("bonds", ctypes.c_void_p),
+        ("nx", ctypes.c_int64), ("ny", ctypes.c_int64), ("nz", ctypes.c_int64),
+        ("x_width", ctypes.c_double), ("y_width", ctypes.c_double), ("z_width", ctypes.c_double),
+        ("bead_vol", ctypes.c_double), ("vf_limit", ctypes.c_double), ("mu_adjust", ctypes.c_double),
+        ("v_int", ctypes.c_double * MAX_BINDERS),
+        ("cross_mask", ctypes.c_int32 * (MAX_BINDERS * MAX_BINDERS)),
+        ("F", ctypes.c_double * (MAX_BINDERS * MAX_MARK * (MAX_SITES + 1))),
+    ]
+
+
 _c_void_p = ctypes.c_void_p
 _c_i32 = ctypes.c_int32
 _c_i64 = ctypes.c_int64
@@ -105,7 +119,12 @@ _SIGNATURES = {
     "chromo_field_dE_tape": (ctypes.c_int, [_c_void_p, _c_i32, _c_void_p, _c_i64, _c_void_p,
                                             _c_void_p, _c_i32, _c_void_p, _c_void_p, _c_void_p,
                                             _c_void_p, _c_void_p]),
-    "chromo_energy_reference": (ctypes.c_int, [_c_void_p, _c_i32, _c_void_p, _c_void_p]),
+    "chromo_energy_adjoint": (ctypes.c_int, [ctypes.POINTER(ChromoEnergyModel), _c_void_p,
+                                             _c_void_p, _c_void_p, _c_void_p, _c_void_p,
+                                             _c_void_p]),
+    "chromo_energy_reference": (ctypes.c_int, [ctypes.POINTER(ChromoEnergyModel), _c_void_p,
+                                               _c_i32, _c_void_p, _c_void_p, ctypes.c_double,
+                                               _c_void_p, _c_void_p, _c_void_p]),
 }
 
 _lib = None

@@ -2,7 +2,7 @@
 
     python -m chromo_amd.build
 
-compiles chromo_amd/csrc/chromo_hip.hip with hipcc into chromo_amd/libchromo_hip.so.
+compiles chromo_amd/csrc/chromo_hip.hip and chromo_energy.hip with hipcc into chromo_amd/libchromo_hip.so.
 -ffp-contract=off keeps every multiply and add separately rounded, as in the
 reference's x86-64 build; packed FP32 instructions are disabled.
 """
@@ -16,7 +16,7 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 ROOT = HERE.parent
-SRC = HERE / "csrc" / "chromo_hip.hip"
+SRCS = [HERE / "csrc" / "chromo_hip.hip", HERE / "csrc" / "chromo_energy.hip"]
 OUT = HERE / "libchromo_hip.so"
 ARCH = os.environ.get("CHROMO_OFFLOAD_ARCH", "gfx950")
 
@@ -36,7 +36,7 @@ def hipcc() -> str:
 
 def build(verbose: bool = False) -> Path:
     tmp = OUT.with_suffix(".so.tmp")
-    cmd = [hipcc(), *FLAGS, "-o", str(tmp), str(SRC)]
+    cmd = [hipcc(), *FLAGS, "-o", str(tmp), *[str(s) for s in SRCS]]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -1,0 +1,245 @@
+// chromo_density.hpp -- deterministic voxel-density scatter shared by the MC
+// initialisation and the energy / adjoint kernels.
+//
+// UniformDensityField.update_all_densities_for_all_polymers
+// (chromo/fields.pyx:2041-2106) adds every bead's eight trilinear weights to
+// the grid in bead order.  To reproduce those per-voxel sums bit for bit in
+// parallel, beads are bucketed by their lower-corner voxel (counting sort with
+// sorted buckets) and every voxel then gathers from the <= 8 buckets that reach
+// it, merging them in ascending bead index.  No floating-point atomics.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "chromo_device.hpp"
+
+namespace chromo {
+namespace {
+
+// Grid of a UniformDensityField (fields.pyx:536-575).
+struct GridGeom {
+    int64_t nx, ny, nz;
+    double width[3], dxyz[3], half_width[3], half_step[3];
+    double vol_bin;
+};
+
+// Bead coordinate c of bead b is p[c * sc + b * sb].
+struct PosView {
+    const double *p;
+    int64_t sc, sb;
+};
+
+__device__ __forceinline__ void grid_lower(const GridGeom &G, double x0, double x1, double x2,
+                                           int64_t ix[3], double w[3])
+{
+    const int64_t nn[3] = {G.nx, G.ny, G.nz};
+    const double xs[3] = {x0, x1, x2};
+    for (int j = 0; j < 3; ++j) {
+        double x = pymod(xs[j] + G.half_width[j], G.width[j]) - G.half_step[j];
+        double q = x / G.dxyz[j];
+        if (!(q >= -1.0 && q < (double)nn[j])) q = 0.0;
+        int64_t ind = (int64_t)floor(q);
+        ix[j] = (ind == -1) ? nn[j] - 1 : ind;
+        w[j] = 1 - (q - (double)ind);
+    }
+}
+
+__global__ void k_dens_cells(GridGeom G, int64_t N, PosView P, int32_t *cell, double *wts,
+                             int32_t *cnt)
+{
+    int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= N) return;
+    int64_t ix[3];
+    double w[3];
+    grid_lower(G, P.p[b * P.sb], P.p[P.sc + b * P.sb], P.p[2 * P.sc + b * P.sb], ix, w);
+    int32_t c = (int32_t)(ix[0] + ix[1] * G.nx + ix[2] * G.nx * G.ny);
+    cell[b] = c;
+    wts[b] = w[0];
+    wts[N + b] = w[1];
+    wts[2 * N + b] = w[2];
+    atomicAdd(&cnt[c], 1);
+}
+
+// exclusive scan: one block of 1024 threads per 2048 elements, block totals in bsum
+__global__ void k_scan_blocks(const int32_t *in, int32_t *out, int32_t *bsum, int64_t n)
+{
+    __shared__ int32_t s[2048];
+    int64_t base = (int64_t)blockIdx.x * 2048;
+    int t = threadIdx.x;
+    for (int i = t; i < 2048; i += 1024) s[i] = (base + i < n) ? in[base + i] : 0;
+    __syncthreads();
+    for (int d = 1; d < 2048; d <<= 1) {
+        int32_t v0 = 0, v1 = 0;
+        int i0 = t, i1 = t + 1024;
+        if (i0 >= d) v0 = s[i0 - d];
+        if (i1 >= d) v1 = s[i1 - d];
+        __syncthreads();
+        s[i0] += v0;
+        s[i1] += v1;
+        __syncthreads();
+    }
+    for (int i = t; i < 2048; i += 1024) {
+        if (base + i < n) out[base + i] = s[i] - in[base + i];
+    }
+    if (t == 0) bsum[blockIdx.x] = s[2047];
+}
+
+__global__ void k_scan_add(int32_t *out, const int32_t *boff, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += boff[i / 2048];
+}
+
+__global__ void k_dens_place(int64_t N, const int32_t *cell, const int32_t *off, int32_t *fill,
+                             int32_t *list)
+{
+    int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= N) return;
+    int32_t c = cell[b];
+    int32_t pos = off[c] + atomicAdd(&fill[c], 1);
+    list[pos] = (int32_t)b;
+}
+
+__global__ void k_dens_sort(int64_t nbins, const int32_t *off, const int32_t *cnt, int32_t *list)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nbins) return;
+    int32_t s0 = off[c], s1 = s0 + cnt[c];
+    for (int32_t a = s0 + 1; a < s1; ++a) {
+        int32_t x = list[a];
+        int32_t q = a - 1;
+        while (q >= s0 && list[q] > x) {
+            list[q + 1] = list[q];
+            --q;
+        }
+        list[q + 1] = x;
+    }
+}
+
+// density[v * Bp1 + m]: m = 0 beads, m >= 1 binder m-1 (weight x state)
+__global__ void k_dens_gather(GridGeom G, int64_t N, int Bp1, const int8_t *states,
+                              const int32_t *off, const int32_t *cnt, const int32_t *list,
+                              const double *wts, double *density, int cleanup)
+{
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nx = G.nx, ny = G.ny, nz = G.nz;
+    if (v >= nx * ny * nz) return;
+    int64_t x = v % nx, y = (v / nx) % ny, z = v / (nx * ny);
+    int32_t cells[8];
+    int masks[8];
+    int nc = 0;
+    for (int l = 0; l < 8; ++l) {
+        int64_t cx = x - (l & 1), cy = y - ((l >> 1) & 1), cz = z - ((l >> 2) & 1);
+        if (cx < 0) cx += nx;
+        if (cy < 0) cy += ny;
+        if (cz < 0) cz += nz;
+        int32_t c = (int32_t)(cx + cy * nx + cz * nx * ny);
+        int f = -1;
+        for (int q = 0; q < nc; ++q)
+            if (cells[q] == c) f = q;
+        if (f < 0) {
+            cells[nc] = c;
+            masks[nc] = 0;
+            f = nc++;
+        }
+        masks[f] |= 1 << l;
+    }
+    int32_t pos[8], end[8];
+    for (int q = 0; q < nc; ++q) {
+        pos[q] = off[cells[q]];
+        end[q] = pos[q] + cnt[cells[q]];
+    }
+    double acc[CHROMO_MAX_BINDERS + 1];
+    for (int m = 0; m < Bp1; ++m) acc[m] = 0;
+    for (;;) {
+        int best = -1;
+        int32_t bb = 0x7fffffff;
+        for (int q = 0; q < nc; ++q)
+            if (pos[q] < end[q] && list[pos[q]] < bb) {
+                bb = list[pos[q]];
+                best = q;
+            }
+        if (best < 0) break;
+        pos[best]++;
+        const double wx = wts[bb], wy = wts[N + bb], wz = wts[2 * N + bb];
+        for (int l = 0; l < 8; ++l) {
+            if (!(masks[best] >> l & 1)) continue;
+            double d = corner_weight(l, wx, wy, wz) / G.vol_bin;
+            acc[0] += d;
+            for (int m = 1; m < Bp1; ++m) acc[m] += d * (double)states[(int64_t)(m - 1) * N + bb];
+        }
+    }
+    for (int m = 0; m < Bp1; ++m) {
+        double a = acc[m];
+        if (cleanup && fabs(a) < 1E-18) a = 0;
+        density[v * Bp1 + m] = a;
+    }
+}
+
+// Scratch for one scatter: sized by the largest chain and grid.
+struct DensityScratch {
+    int32_t *cell = nullptr, *list = nullptr, *cnt = nullptr, *off = nullptr, *fill = nullptr;
+    int32_t *bsum = nullptr, *boff = nullptr, *bsum2 = nullptr;
+    double *wts = nullptr;
+    int64_t maxN = 0, maxBins = 0;
+};
+
+inline hipError_t density_scratch_alloc(DensityScratch &S, int64_t maxN, int64_t maxBins,
+                                        hipStream_t st)
+{
+    hipError_t e;
+    S.maxN = maxN;
+    S.maxBins = maxBins;
+#define CHROMO_ALLOC(ptr, bytes) \
+    if ((e = hipMallocAsync((void **)&ptr, bytes, st)) != hipSuccess) return e
+    CHROMO_ALLOC(S.cell, sizeof(int32_t) * maxN);
+    CHROMO_ALLOC(S.list, sizeof(int32_t) * maxN);
+    CHROMO_ALLOC(S.wts, sizeof(double) * 3 * maxN);
+    CHROMO_ALLOC(S.cnt, sizeof(int32_t) * maxBins);
+    CHROMO_ALLOC(S.off, sizeof(int32_t) * maxBins);
+    CHROMO_ALLOC(S.fill, sizeof(int32_t) * maxBins);
+    CHROMO_ALLOC(S.bsum, sizeof(int32_t) * 2048);
+    CHROMO_ALLOC(S.boff, sizeof(int32_t) * 2048);
+    CHROMO_ALLOC(S.bsum2, sizeof(int32_t) * 2);
+#undef CHROMO_ALLOC
+    return hipSuccess;
+}
+
+inline void density_scratch_free(DensityScratch &S, hipStream_t st)
+{
+    int32_t *ip[] = {S.cell, S.list, S.cnt, S.off, S.fill, S.bsum, S.boff, S.bsum2};
+    for (auto p : ip)
+        if (p) (void)hipFreeAsync(p, st);
+    if (S.wts) (void)hipFreeAsync(S.wts, st);
+    S = DensityScratch();
+}
+
+// Scatter N beads into density (n_bins x Bp1).  Requires N <= S.maxN,
+// n_bins <= S.maxBins <= 2048 * 2048.
+inline hipError_t density_scatter(const GridGeom &G, int64_t N, int Bp1, PosView P,
+                                  const int8_t *states, double *density, int cleanup,
+                                  DensityScratch &S, hipStream_t st)
+{
+    const int64_t nb = G.nx * G.ny * G.nz;
+    hipError_t e;
+    if ((e = hipMemsetAsync(S.cnt, 0, sizeof(int32_t) * nb, st)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(S.fill, 0, sizeof(int32_t) * nb, st)) != hipSuccess) return e;
+    unsigned gb = (unsigned)((N + 255) / 256);
+    unsigned gv = (unsigned)((nb + 255) / 256);
+    hipLaunchKernelGGL(k_dens_cells, dim3(gb), dim3(256), 0, st, G, N, P, S.cell, S.wts, S.cnt);
+    int64_t nbl = (nb + 2047) / 2048;
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nbl), dim3(1024), 0, st, S.cnt, S.off, S.bsum, nb);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, S.bsum, S.boff, S.bsum2, nbl);
+    hipLaunchKernelGGL(k_scan_add, dim3(gv), dim3(256), 0, st, S.off, S.boff, nb);
+    hipLaunchKernelGGL(k_dens_place, dim3(gb), dim3(256), 0, st, N, S.cell, S.off, S.fill, S.list);
+    hipLaunchKernelGGL(k_dens_sort, dim3(gv), dim3(256), 0, st, nb, S.off, S.cnt, S.list);
+    hipLaunchKernelGGL(k_dens_gather, dim3(gv), dim3(256), 0, st, G, N, Bp1, states, S.off, S.cnt,
+                       S.list, S.wts, density, cleanup);
+    return hipGetLastError();
+}
+
+}  // namespace
+}  // namespace chromo

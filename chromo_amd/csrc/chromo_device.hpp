@@ -13,9 +13,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "chromo_hip.h"
 
 namespace chromo {
+
+// Records the message returned by chromo_last_error (this thread) and returns code.
+int set_last_error(int code, const std::string &msg);
 
 constexpr int WAVE = 64;
 constexpr int SEG_LDS = 32;                 // moves with n_inds <= SEG_LDS use the LDS arena

@@ -1,0 +1,530 @@
+// chromo_energy.hip -- absolute energies and their adjoint (gfx950).
+//
+// The Monte Carlo dE of chromo (chromo/polymers.pyx:1024-1538,
+// chromo/fields.pyx:1149-1842) are differences of
+//
+//   E(r, theta) = E_elastic(r)                       sum over bonds of E_pair
+//               + E_field(rho(r), theta)             voxel sums below
+//               + E_bind(states, theta)              single-site free energies
+//
+//   E_field = sum_v [ sum_j pref_j rho_vj^2 + sum_jk X_jk rho_vj rho_vk + g(phi_v) ]
+//             + sum_j intra_j D_j
+//   pref_j = eps_jj v_j Vbin / 2,  X_jk = eps_jk v_j Vbin (cross-talk),
+//   intra_j = eps_jj (1 - v_j / Vbin),  phi_v = rho_v0 V,
+//   g(phi) = chi (Vbin / V) phi^2  (phi <= vf_limit)  else 1e99 phi,
+//   E_bind = sum_b sum_j [ F_j(marks, s) - s mu_j f ]  (mu_j > 0: - s mu_j (2 - f)).
+//
+// (The reference's own compute_E uses chi phi (1 - phi) and omits cross-talk,
+// so its differences are not the move dE; chromo_energy_reference reproduces
+// that function separately.)  Gradients: with respect to bead positions
+// through the trilinear weights (dw/dx = -+1/dx) and the SSWLC bond energy, and
+// with respect to theta = (chi, eps_ij for i <= j, mu_j).  Densities are
+// recomputed (deterministic scatter) rather than stored; all sums use fixed-
+// order block reductions.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <string>
+
+#include "chromo_density.hpp"
+#include "chromo_device.hpp"
+
+using namespace chromo;
+
+namespace {
+
+constexpr int EB = 256;   // block size of the energy kernels
+constexpr int MAXT = 64;  // partial-sum slots per block
+
+int eset(int code, const std::string &m) { return chromo::set_last_error(code, m); }
+
+#define HIPE(x)                                                                            \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) return eset(-2, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct EModel {
+    int64_t N;
+    int B;
+    GridGeom G;
+    const double *t3;
+    const int8_t *states, *marks;
+    const double *bonds;
+    double bead_vol, vf_limit, mu_adjust;
+    double v_int[CHROMO_MAX_BINDERS];
+    int cross_mask[CHROMO_MAX_BINDERS * CHROMO_MAX_BINDERS];
+    double F[CHROMO_MAX_BINDERS][CHROMO_MAX_MARK][CHROMO_MAX_SITES + 1];
+};
+
+// theta layout: [chi, eps_00, eps_01, .., eps_0B-1, eps_11, .., eps_B-1B-1, mu_0 .. mu_B-1]
+__host__ __device__ inline int eps_index(int B, int i, int j)
+{
+    if (i > j) {
+        int t = i;
+        i = j;
+        j = t;
+    }
+    // rows 0..i-1 hold B, B-1, .. entries
+    return 1 + i * B - (i * (i - 1)) / 2 + (j - i);
+}
+
+__host__ __device__ inline int n_theta(int B) { return 1 + B * (B + 1) / 2 + B; }
+
+struct Coef {
+    double chi;
+    double pref[CHROMO_MAX_BINDERS], intra[CHROMO_MAX_BINDERS];
+    double X[CHROMO_MAX_BINDERS * CHROMO_MAX_BINDERS];
+    double mu[CHROMO_MAX_BINDERS];
+};
+
+__device__ inline Coef coef_of(const EModel &M, const double *th)
+{
+    Coef c;
+    const int B = M.B;
+    const double vb = M.G.vol_bin;
+    c.chi = th[0];
+    for (int j = 0; j < B; ++j) {
+        double e = th[eps_index(B, j, j)];
+        c.pref[j] = 0.5 * e * M.v_int[j] * vb;
+        c.intra[j] = e * (1 - M.v_int[j] / vb);
+        c.mu[j] = th[1 + B * (B + 1) / 2 + j];
+        for (int k = 0; k < B; ++k)
+            c.X[j * B + k] = M.cross_mask[j * B + k] ? th[eps_index(B, j, k)] * M.v_int[j] * vb : 0.0;
+    }
+    return c;
+}
+
+// fixed-order block sum of MAXT-wide partial vectors
+__device__ void block_reduce_store(double *vals, int nt, double *out_block)
+{
+    __shared__ double sh[EB];
+    for (int t = 0; t < nt; ++t) {
+        sh[threadIdx.x] = vals[t];
+        __syncthreads();
+        for (int s = EB / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out_block[t] = sh[0];
+        __syncthreads();
+    }
+}
+
+// ---- voxel terms -----------------------------------------------------------
+// slots: [0..B) S_jj, [B..B+B(B-1)/2) S_jk (j<k), then sum g, sum dg/dchi
+__global__ void k_e_voxel(EModel M, const double *theta, const double *rho, double *dEdrho,
+                          double *part, int nt)
+{
+    const int B = M.B, Bp1 = B + 1;
+    const Coef c = coef_of(M, theta);
+    const int64_t nb = M.G.nx * M.G.ny * M.G.nz;
+    double acc[MAXT];
+    for (int t = 0; t < nt; ++t) acc[t] = 0;
+    const double V = M.bead_vol, vb = M.G.vol_bin;
+    for (int64_t v = (int64_t)blockIdx.x * EB + threadIdx.x; v < nb; v += (int64_t)gridDim.x * EB) {
+        const double *r = rho + v * Bp1;
+        int s = B;
+        for (int j = 0; j < B; ++j) {
+            acc[j] += r[j + 1] * r[j + 1];
+            for (int k = j + 1; k < B; ++k) acc[s++] += r[j + 1] * r[k + 1];
+        }
+        double phi = r[0] * V;
+        bool huge = phi > M.vf_limit;
+        acc[s] += huge ? E_HUGE_FIELD * phi : c.chi * (vb / V) * (phi * phi);
+        acc[s + 1] += huge ? 0.0 : (vb / V) * (phi * phi);
+        if (dEdrho) {
+            double *d = dEdrho + v * Bp1;
+            d[0] = V * (huge ? E_HUGE_FIELD : 2 * c.chi * (vb / V) * phi);
+            for (int j = 0; j < B; ++j) {
+                double g = 2 * c.pref[j] * r[j + 1];
+                for (int k = 0; k < B; ++k) g += (c.X[j * B + k] + c.X[k * B + j]) * r[k + 1];
+                d[j + 1] = g;
+            }
+        }
+    }
+    block_reduce_store(acc, nt, part + (int64_t)blockIdx.x * MAXT);
+}
+
+// ---- bead / bond terms -------------------------------------------------------
+__device__ inline double E_pair_abs(const Bond &bd, const double *r0, const double *r1,
+                                    const double *t30, const double *t31)
+{
+    double dr[3], drp[3], be[3];
+    for (int i = 0; i < 3; ++i) dr[i] = r1[i] - r0[i];
+    double par = dot3(t30, dr);
+    for (int i = 0; i < 3; ++i) {
+        drp[i] = dr[i] - t30[i] * par;
+        be[i] = t31[i] - t30[i] - drp[i] * bd.eta;
+    }
+    return E_pair(bd, be, par, drp);
+}
+
+// dE_pair / d(r1 - r0) with tangents fixed
+__device__ inline void E_pair_grad_dr(const Bond &bd, const double *r0, const double *r1,
+                                      const double *t30, const double *t31, double g[3])
+{
+    double dr[3], drp[3], be[3];
+    for (int i = 0; i < 3; ++i) dr[i] = r1[i] - r0[i];
+    double par = dot3(t30, dr);
+    for (int i = 0; i < 3; ++i) {
+        drp[i] = dr[i] - t30[i] * par;
+        be[i] = t31[i] - t30[i] - drp[i] * bd.eta;
+    }
+    // perp = P dr with P = I - t t^T; bend = t31 - t30 - eta perp
+    double a[3];
+    for (int i = 0; i < 3; ++i) a[i] = bd.eperp * drp[i] - bd.eta * bd.eb * be[i];
+    double ta = dot3(t30, a);
+    for (int i = 0; i < 3; ++i) g[i] = (a[i] - t30[i] * ta) + bd.ep * (par - bd.gam) * t30[i];
+}
+
+// slots: [0] E_elastic, [1] E_bind, [2..2+B) D_j, [2+B..2+2B) sum s_j
+__global__ void k_e_bead(EModel M, const double *theta, const double *r, const double *dEdrho,
+                         double *grad, double *part, int nt)
+{
+    const int B = M.B, Bp1 = B + 1;
+    const int64_t N = M.N;
+    const Coef c = coef_of(M, theta);
+    double acc[MAXT];
+    for (int t = 0; t < nt; ++t) acc[t] = 0;
+    const double f = M.mu_adjust;
+    for (int64_t b = (int64_t)blockIdx.x * EB + threadIdx.x; b < N; b += (int64_t)gridDim.x * EB) {
+        const double *rb = r + 3 * b;
+        const double *tb = M.t3 + 3 * b;
+        if (b + 1 < N) {
+            Bond bd = load_bond(M.bonds, N, b);
+            acc[0] += E_pair_abs(bd, rb, r + 3 * (b + 1), tb, M.t3 + 3 * (b + 1));
+        }
+        for (int j = 0; j < B; ++j) {
+            int s = M.states[(int64_t)j * N + b];
+            int m = M.marks[(int64_t)j * N + b];
+            double mu = c.mu[j];
+            double e = M.F[j][m][s];
+            e += (mu > 0) ? -(double)s * (mu * (2 - f)) : -(double)s * mu * f;
+            acc[1] += e;
+            acc[2 + j] += (s == 2);
+            acc[2 + B + j] += (double)s * ((mu > 0) ? (2 - f) : f);
+        }
+        if (grad) {
+            double g[3] = {0, 0, 0};
+            if (dEdrho) {
+                int64_t ix[3];
+                double w[3];
+                grid_lower(M.G, rb[0], rb[1], rb[2], ix, w);
+                for (int l = 0; l < 8; ++l) {
+                    int64_t x = ix[0] + (l & 1), y = ix[1] + ((l >> 1) & 1), z = ix[2] + ((l >> 2) & 1);
+                    if (x >= M.G.nx) x -= M.G.nx;
+                    if (y >= M.G.ny) y -= M.G.ny;
+                    if (z >= M.G.nz) z -= M.G.nz;
+                    int64_t v = x + y * M.G.nx + z * M.G.nx * M.G.ny;
+                    const double *d = dEdrho + v * Bp1;
+                    double h = d[0];
+                    for (int j = 0; j < B; ++j) h += (double)M.states[(int64_t)j * N + b] * d[j + 1];
+                    h /= M.G.vol_bin;
+                    double a[3];
+                    for (int j = 0; j < 3; ++j) a[j] = ((l >> j) & 1) ? (1 - w[j]) : w[j];
+                    for (int j = 0; j < 3; ++j) {
+                        double dw = (((l >> j) & 1) ? 1.0 : -1.0) / M.G.dxyz[j];
+                        double prod = dw;
+                        for (int q = 0; q < 3; ++q)
+                            if (q != j) prod *= a[q];
+                        g[j] += prod * h;
+                    }
+                }
+            }
+            if (b + 1 < N) {   // bond (b, b+1): r0 = r_b
+                Bond bd = load_bond(M.bonds, N, b);
+                double gd[3];
+                E_pair_grad_dr(bd, rb, r + 3 * (b + 1), tb, M.t3 + 3 * (b + 1), gd);
+                for (int j = 0; j < 3; ++j) g[j] -= gd[j];
+            }
+            if (b > 0) {       // bond (b-1, b): r1 = r_b
+                Bond bd = load_bond(M.bonds, N, b - 1);
+                double gd[3];
+                E_pair_grad_dr(bd, r + 3 * (b - 1), rb, M.t3 + 3 * (b - 1), tb, gd);
+                for (int j = 0; j < 3; ++j) g[j] += gd[j];
+            }
+            for (int j = 0; j < 3; ++j) grad[3 * b + j] = g[j];
+        }
+    }
+    block_reduce_store(acc, nt, part + (int64_t)blockIdx.x * MAXT);
+}
+
+// Final single-block reduction over block partials (fixed order) and assembly.
+__global__ void k_e_final(EModel M, const double *theta, const double *vpart, int nbv, int ntv,
+                          const double *bpart, int nbb, int ntb, double *E_out, double *gtheta)
+{
+    __shared__ double sv[MAXT], sb[MAXT];
+    int t = threadIdx.x;
+    if (t < ntv) {
+        double s = 0;
+        for (int i = 0; i < nbv; ++i) s += vpart[(int64_t)i * MAXT + t];
+        sv[t] = s;
+    }
+    if (t < ntb) {
+        double s = 0;
+        for (int i = 0; i < nbb; ++i) s += bpart[(int64_t)i * MAXT + t];
+        sb[t] = s;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    const int B = M.B;
+    const Coef c = coef_of(M, theta);
+    const double vb = M.G.vol_bin;
+    double S[CHROMO_MAX_BINDERS * CHROMO_MAX_BINDERS];
+    int s = B;
+    for (int j = 0; j < B; ++j) {
+        S[j * B + j] = sv[j];
+        for (int k = j + 1; k < B; ++k) {
+            S[j * B + k] = sv[s];
+            S[k * B + j] = sv[s];
+            ++s;
+        }
+    }
+    const double Eg = sv[s], dchi = sv[s + 1];
+    double Ef = 0;
+    for (int j = 0; j < B; ++j) {
+        Ef += c.pref[j] * S[j * B + j];
+        Ef += c.intra[j] * sb[2 + j];
+        for (int k = 0; k < B; ++k) Ef += c.X[j * B + k] * S[j * B + k];
+    }
+    Ef += Eg;
+    const double Eel = sb[0], Ebd = sb[1];
+    E_out[0] = Eel + Ef + Ebd;
+    E_out[1] = Eel;
+    E_out[2] = Ef;
+    E_out[3] = Ebd;
+    if (gtheta) {
+        gtheta[0] = dchi;
+        for (int j = 0; j < B; ++j)
+            for (int k = j; k < B; ++k) {
+                double g;
+                if (j == k) {
+                    g = 0.5 * M.v_int[j] * vb * S[j * B + j] + (1 - M.v_int[j] / vb) * sb[2 + j];
+                    if (M.cross_mask[j * B + j]) g += M.v_int[j] * vb * S[j * B + j];
+                } else {
+                    g = ((M.cross_mask[j * B + k] ? M.v_int[j] : 0.0) +
+                         (M.cross_mask[k * B + j] ? M.v_int[k] : 0.0)) * vb * S[j * B + k];
+                }
+                gtheta[eps_index(B, j, k)] = g;
+            }
+        for (int j = 0; j < B; ++j) gtheta[1 + B * (B + 1) / 2 + j] = -sb[2 + B + j];
+    }
+}
+
+// ---- the reference's compute_E (fields.pyx:1939-2286, polymers.pyx:1348-1381) ----
+
+// Python round(x, 2) > lim, evaluated on the exact binary value of x.
+__device__ inline double py_round2(double x)
+{
+    double p = x * 100.0;
+    double e = fma(x, 100.0, -p);   // exact residual: x * 100 = p + e
+    double k = rint(p);
+    double d = p - k;
+    if (d == 0.5 && e > 0) k += 1.0;
+    else if (d == -0.5 && e < 0) k -= 1.0;
+    return k / 100.0;
+}
+
+// slots: [0..B) sum rho_j^2, [B] nonspecific E
+__global__ void k_ref_voxel(EModel M, const double *pref, double chi, const double *rho, double *part,
+                            int nt)
+{
+    const int B = M.B, Bp1 = B + 1;
+    const int64_t nb = M.G.nx * M.G.ny * M.G.nz;
+    double acc[MAXT];
+    for (int t = 0; t < nt; ++t) acc[t] = 0;
+    const double V = M.bead_vol, vb = M.G.vol_bin;
+    for (int64_t v = (int64_t)blockIdx.x * EB + threadIdx.x; v < nb; v += (int64_t)gridDim.x * EB) {
+        const double *r = rho + v * Bp1;
+        for (int j = 0; j < B; ++j) acc[j] += r[j + 1] * r[j + 1];
+        double vf = r[0] * V;
+        if (py_round2(vf) > M.vf_limit) acc[B] += E_HUGE_FIELD * vf;
+        else acc[B] += chi * (vb / V) * vf * (1 - vf);
+    }
+    block_reduce_store(acc, nt, part + (int64_t)blockIdx.x * MAXT);
+}
+
+// slots: [0] SSWLC.compute_E bonds, [1..1+B) doubly bound counts
+__global__ void k_ref_bead(EModel M, const double *pos, int64_t sc, int64_t sb, double *part, int nt)
+{
+    const int B = M.B;
+    const int64_t N = M.N;
+    double acc[MAXT];
+    for (int t = 0; t < nt; ++t) acc[t] = 0;
+    for (int64_t b = (int64_t)blockIdx.x * EB + threadIdx.x; b < N; b += (int64_t)gridDim.x * EB) {
+        if (b + 1 < N) {
+            double r0[3], r1[3], t30[3], t31[3];
+            for (int j = 0; j < 3; ++j) {
+                r0[j] = pos[j * sc + b * sb];
+                r1[j] = pos[j * sc + (b + 1) * sb];
+                t30[j] = pos[(3 + j) * sc + b * sb];
+                t31[j] = pos[(3 + j) * sc + (b + 1) * sb];
+            }
+            Bond bd = load_bond(M.bonds, N, b);
+            double dr[3], drp[3], be[3];
+            for (int j = 0; j < 3; ++j) dr[j] = r1[j] - r0[j];
+            double par = dot3(t30, dr);
+            for (int j = 0; j < 3; ++j) drp[j] = dr[j] - t30[j] * par;
+            for (int j = 0; j < 3; ++j) {   // bend = t3_1; bend += -t3_0 - eta dr_perp
+                be[j] = t31[j];
+                be[j] += -t30[j] - bd.eta * drp[j];
+            }
+            acc[0] += E_pair(bd, be, par, drp);
+        }
+        for (int j = 0; j < B; ++j) acc[1 + j] += (M.states[(int64_t)j * N + b] == 2);
+    }
+    block_reduce_store(acc, nt, part + (int64_t)blockIdx.x * MAXT);
+}
+
+__global__ void k_ref_final(EModel M, const double *pref, const double *intra, const double *vpart,
+                            int nbv, const double *bpart, int nbb, double *out)
+{
+    __shared__ double sv[MAXT], sb[MAXT];
+    int t = threadIdx.x;
+    const int B = M.B;
+    if (t < B + 1) {
+        double s = 0;
+        for (int i = 0; i < nbv; ++i) s += vpart[(int64_t)i * MAXT + t];
+        sv[t] = s;
+    }
+    if (t < B + 1) {
+        double s = 0;
+        for (int i = 0; i < nbb; ++i) s += bpart[(int64_t)i * MAXT + t];
+        sb[t] = s;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    double E = 0;
+    for (int i = 0; i < B; ++i) {
+        E += pref[i] * sv[i];
+        E += intra[i] * sb[1 + i];
+    }
+    E += sv[B];
+    out[0] = sb[0];
+    out[1] = E;
+}
+
+int grid_blocks(int64_t n)
+{
+    int64_t g = (n + EB - 1) / EB;
+    if (g > 1024) g = 1024;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+int make_model(const chromo_energy_model *m, EModel &M)
+{
+    if (m == nullptr) return eset(-1, "model is NULL");
+    if (m->B < 1 || m->B > CHROMO_MAX_BINDERS) return eset(-1, "B outside 1..4");
+    if (m->N < 2) return eset(-1, "N < 2");
+    if (n_theta(m->B) > MAXT || 2 + 2 * m->B > MAXT) return eset(-1, "too many terms");
+    M.N = m->N;
+    M.B = m->B;
+    M.G.nx = m->nx;
+    M.G.ny = m->ny;
+    M.G.nz = m->nz;
+    const double W[3] = {m->x_width, m->y_width, m->z_width};
+    const int64_t n[3] = {m->nx, m->ny, m->nz};
+    for (int j = 0; j < 3; ++j) {
+        M.G.width[j] = W[j];
+        M.G.dxyz[j] = W[j] / (double)n[j];
+        M.G.half_width[j] = 0.5 * W[j];
+        M.G.half_step[j] = 0.5 * M.G.dxyz[j];
+    }
+    M.G.vol_bin = m->x_width * m->y_width * m->z_width / (double)(m->nx * m->ny * m->nz);
+    M.t3 = m->t3;
+    M.states = m->states;
+    M.marks = m->marks;
+    M.bonds = m->bonds;
+    M.bead_vol = m->bead_vol;
+    M.vf_limit = m->vf_limit;
+    M.mu_adjust = m->mu_adjust;
+    for (int j = 0; j < CHROMO_MAX_BINDERS; ++j) M.v_int[j] = m->v_int[j];
+    for (int j = 0; j < CHROMO_MAX_BINDERS * CHROMO_MAX_BINDERS; ++j) M.cross_mask[j] = 0;
+    for (int j = 0; j < m->B; ++j)
+        for (int k = 0; k < m->B; ++k) M.cross_mask[j * m->B + k] = m->cross_mask[j * CHROMO_MAX_BINDERS + k];
+    memcpy(M.F, m->F, sizeof(M.F));
+    int64_t nb = m->nx * m->ny * m->nz;
+    if (nb < 1 || nb > 2048LL * 2048LL) return eset(-1, "grid size outside 1..2^22 voxels");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int chromo_energy_adjoint(const chromo_energy_model *model, const double *d_r, const double *d_theta,
+                          double *d_E, double *d_grad_r, double *d_grad_theta, void *stream)
+{
+    EModel M;
+    int rc = make_model(model, M);
+    if (rc) return rc;
+    if (!d_r || !d_theta || !d_E) return eset(-1, "NULL argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int B = M.B, Bp1 = B + 1;
+    const int64_t nb = M.G.nx * M.G.ny * M.G.nz;
+    double *rho = nullptr, *dEdrho = nullptr, *vpart = nullptr, *bpart = nullptr;
+    DensityScratch S;
+    HIPE(density_scratch_alloc(S, M.N, nb, st));
+    HIPE(hipMallocAsync((void **)&rho, sizeof(double) * nb * Bp1, st));
+    if (d_grad_r) HIPE(hipMallocAsync((void **)&dEdrho, sizeof(double) * nb * Bp1, st));
+    const int gv = grid_blocks(nb), gb = grid_blocks(M.N);
+    HIPE(hipMallocAsync((void **)&vpart, sizeof(double) * MAXT * gv, st));
+    HIPE(hipMallocAsync((void **)&bpart, sizeof(double) * MAXT * gb, st));
+    PosView P{d_r, 1, 3};
+    hipError_t e = density_scatter(M.G, M.N, Bp1, P, M.states, rho, 0, S, st);
+    if (e != hipSuccess) return eset(-2, std::string("density scatter: ") + hipGetErrorString(e));
+    const int ntv = B + B * (B - 1) / 2 + 2, ntb = 2 + 2 * B;
+    hipLaunchKernelGGL(k_e_voxel, dim3(gv), dim3(EB), 0, st, M, d_theta, rho, dEdrho, vpart, ntv);
+    hipLaunchKernelGGL(k_e_bead, dim3(gb), dim3(EB), 0, st, M, d_theta, d_r, dEdrho, d_grad_r, bpart, ntb);
+    hipLaunchKernelGGL(k_e_final, dim3(1), dim3(64), 0, st, M, d_theta, vpart, gv, ntv, bpart, gb, ntb,
+                       d_E, d_grad_theta);
+    HIPE(hipGetLastError());
+    density_scratch_free(S, st);
+    HIPE(hipFreeAsync(rho, st));
+    if (dEdrho) HIPE(hipFreeAsync(dEdrho, st));
+    HIPE(hipFreeAsync(vpart, st));
+    HIPE(hipFreeAsync(bpart, st));
+    return 0;
+}
+
+int chromo_energy_reference(const chromo_energy_model *model, const double *d_pos, int32_t soa,
+                            const double *d_pref, const double *d_intra, double chi,
+                            double *d_density_out, double *d_out, void *stream)
+{
+    EModel M;
+    int rc = make_model(model, M);
+    if (rc) return rc;
+    if (!d_pos || !d_pref || !d_intra || !d_out) return eset(-1, "NULL argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int B = M.B, Bp1 = B + 1;
+    const int64_t nb = M.G.nx * M.G.ny * M.G.nz;
+    double *rho = d_density_out, *vpart = nullptr, *bpart = nullptr;
+    DensityScratch S;
+    HIPE(density_scratch_alloc(S, M.N, nb, st));
+    if (!rho) HIPE(hipMallocAsync((void **)&rho, sizeof(double) * nb * Bp1, st));
+    const int gv = grid_blocks(nb), gb = grid_blocks(M.N);
+    HIPE(hipMallocAsync((void **)&vpart, sizeof(double) * MAXT * gv, st));
+    HIPE(hipMallocAsync((void **)&bpart, sizeof(double) * MAXT * gb, st));
+    // UniformDensityField.update_all_densities: bead-order scatter without the 1e-18 clean-up
+    const int64_t sc = soa ? M.N : 1, sbd = soa ? 1 : 9;
+    PosView P{d_pos, sc, sbd};
+    hipError_t e = density_scatter(M.G, M.N, Bp1, P, M.states, rho, 0, S, st);
+    if (e != hipSuccess) return eset(-2, std::string("density scatter: ") + hipGetErrorString(e));
+    hipLaunchKernelGGL(k_ref_voxel, dim3(gv), dim3(EB), 0, st, M, d_pref, chi, rho, vpart, B + 1);
+    hipLaunchKernelGGL(k_ref_bead, dim3(gb), dim3(EB), 0, st, M, d_pos, sc, sbd, bpart, B + 1);
+    hipLaunchKernelGGL(k_ref_final, dim3(1), dim3(64), 0, st, M, d_pref, d_intra, vpart, gv, bpart, gb,
+                       d_out);
+    HIPE(hipGetLastError());
+    density_scratch_free(S, st);
+    if (!d_density_out) HIPE(hipFreeAsync(rho, st));
+    HIPE(hipFreeAsync(vpart, st));
+    HIPE(hipFreeAsync(bpart, st));
+    return 0;
+}
+
+}  // extern "C"

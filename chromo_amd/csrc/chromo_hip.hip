@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "chromo_device.hpp"
+#include "chromo_density.hpp"
 
 using namespace chromo;
 
@@ -1127,148 +1128,22 @@ __global__ void k_scratch_init(const chromo_replica *reps, int n_reps)
     }
 }
 
-// ---- deterministic density initialisation ---------------------------------
-// update_all_densities_for_all_polymers (fields.pyx:2041-2106) scatters beads
-// in index order.  Here: bucket beads by their lower-corner cell, sort each
-// bucket, and let every voxel gather from the (up to 8) buckets that reach it
-// in ascending bead order, which reproduces the reference's per-voxel fold.
+// ---- deterministic density initialisation: see chromo_density.hpp ---------
 
-__global__ void k_init_cells(const chromo_replica *R, int32_t *cell, double *wts, int32_t *cnt)
+GridGeom grid_of(const chromo_replica &r)
 {
-    int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= R->N) return;
-    const int64_t N = R->N;
-    double p[3] = {R->pos[b], R->pos[N + b], R->pos[2 * N + b]};
-    int64_t ix[3];
-    double w[3];
-    voxel_lower(R, p, ix, w);
-    int32_t c = (int32_t)(ix[0] + ix[1] * R->nx + ix[2] * R->nx * R->ny);
-    cell[b] = c;
-    wts[b] = w[0];
-    wts[N + b] = w[1];
-    wts[2 * N + b] = w[2];
-    atomicAdd(&cnt[c], 1);
-}
-
-// exclusive scan, one block of 1024 threads per 2048 elements + block sums
-__global__ void k_scan_blocks(const int32_t *in, int32_t *out, int32_t *bsum, int64_t n)
-{
-    __shared__ int32_t s[2048];
-    int64_t base = (int64_t)blockIdx.x * 2048;
-    int t = threadIdx.x;
-    for (int i = t; i < 2048; i += 1024) s[i] = (base + i < n) ? in[base + i] : 0;
-    __syncthreads();
-    for (int d = 1; d < 2048; d <<= 1) {
-        int32_t v0 = 0, v1 = 0;
-        int i0 = t, i1 = t + 1024;
-        if (i0 >= d) v0 = s[i0 - d];
-        if (i1 >= d) v1 = s[i1 - d];
-        __syncthreads();
-        s[i0] += v0;
-        s[i1] += v1;
-        __syncthreads();
+    GridGeom G;
+    G.nx = r.nx;
+    G.ny = r.ny;
+    G.nz = r.nz;
+    for (int j = 0; j < 3; ++j) {
+        G.width[j] = r.width[j];
+        G.dxyz[j] = r.dxyz[j];
+        G.half_width[j] = r.half_width[j];
+        G.half_step[j] = r.half_step[j];
     }
-    for (int i = t; i < 2048; i += 1024) {
-        int32_t incl = s[i];
-        int32_t x = (base + i < n) ? in[base + i] : 0;
-        if (base + i < n) out[base + i] = incl - x;
-    }
-    if (t == 0) bsum[blockIdx.x] = s[2047];
-}
-
-__global__ void k_scan_add(int32_t *out, const int32_t *boff, int64_t n)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] += boff[i / 2048];
-}
-
-__global__ void k_init_place(const chromo_replica *R, const int32_t *cell, const int32_t *off,
-                             int32_t *fill, int32_t *list)
-{
-    int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= R->N) return;
-    int32_t c = cell[b];
-    int32_t pos = off[c] + atomicAdd(&fill[c], 1);
-    list[pos] = (int32_t)b;
-}
-
-__global__ void k_init_sort(int64_t nbins, const int32_t *off, const int32_t *cnt, int32_t *list)
-{
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nbins) return;
-    int32_t s0 = off[c], s1 = s0 + cnt[c];
-    for (int32_t a = s0 + 1; a < s1; ++a) {
-        int32_t x = list[a];
-        int32_t q = a - 1;
-        while (q >= s0 && list[q] > x) {
-            list[q + 1] = list[q];
-            --q;
-        }
-        list[q + 1] = x;
-    }
-}
-
-__global__ void k_init_gather(const chromo_replica *R, const int32_t *off, const int32_t *cnt,
-                              const int32_t *list, const double *wts, int cleanup)
-{
-    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nx = R->nx, ny = R->ny, nz = R->nz;
-    const int64_t nb = nx * ny * nz;
-    if (v >= nb) return;
-    const int64_t N = R->N;
-    const int B = R->B, Bp1 = B + 1;
-    int64_t x = v % nx, y = (v / nx) % ny, z = v / (nx * ny);
-    // distinct source cells and, for each, the corner mask reaching v
-    int32_t cells[8];
-    int masks[8];
-    int nc = 0;
-    for (int l = 0; l < 8; ++l) {
-        int64_t cx = x - (l & 1), cy = y - ((l >> 1) & 1), cz = z - ((l >> 2) & 1);
-        if (cx < 0) cx += nx;
-        if (cy < 0) cy += ny;
-        if (cz < 0) cz += nz;
-        int32_t c = (int32_t)(cx + cy * nx + cz * nx * ny);
-        int f = -1;
-        for (int q = 0; q < nc; ++q)
-            if (cells[q] == c) f = q;
-        if (f < 0) {
-            cells[nc] = c;
-            masks[nc] = 0;
-            f = nc++;
-        }
-        masks[f] |= 1 << l;
-    }
-    int32_t pos[8], end[8];
-    for (int q = 0; q < nc; ++q) {
-        pos[q] = off[cells[q]];
-        end[q] = pos[q] + cnt[cells[q]];
-    }
-    double acc[CHROMO_MAX_BINDERS + 1];
-    for (int m = 0; m < Bp1; ++m) acc[m] = 0;
-    const double vol_bin = R->vol_bin;
-    for (;;) {
-        int best = -1;
-        int32_t bb = 0x7fffffff;
-        for (int q = 0; q < nc; ++q)
-            if (pos[q] < end[q] && list[pos[q]] < bb) {
-                bb = list[pos[q]];
-                best = q;
-            }
-        if (best < 0) break;
-        pos[best]++;
-        const double wx = wts[bb], wy = wts[N + bb], wz = wts[2 * N + bb];
-        for (int l = 0; l < 8; ++l) {
-            if (!(masks[best] >> l & 1)) continue;
-            double d = corner_weight(l, wx, wy, wz) / vol_bin;
-            acc[0] += d;
-            for (int m = 1; m < Bp1; ++m) acc[m] += d * (double)R->states[(int64_t)(m - 1) * N + bb];
-        }
-    }
-    for (int m = 0; m < Bp1; ++m) {
-        double a = acc[m];
-        if (cleanup && fabs(a) < 1E-18) a = 0;
-        R->density[v * Bp1 + m] = a;
-    }
+    G.vol_bin = r.vol_bin;
+    return G;
 }
 
 // ---- tape: field dE of a host-supplied trial configuration ----------------
@@ -1477,13 +1352,7 @@ __global__ void __launch_bounds__(64) k_field_tape(const chromo_replica *reps, i
 // Host side
 // ---------------------------------------------------------------------------
 
-thread_local std::string g_err;
-
-int set_err(int code, const std::string &m)
-{
-    g_err = m;
-    return code;
-}
+int set_err(int code, const std::string &m) { return chromo::set_last_error(code, m); }
 
 #define HIPC(x)                                                                          \
     do {                                                                                 \
@@ -1527,11 +1396,21 @@ int ensure_lds()
 
 }  // namespace
 
+namespace chromo {
+thread_local std::string g_last_error;
+int set_last_error(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+}  // namespace chromo
+
 extern "C" {
 
 int chromo_last_error(char *buf, size_t len)
 {
     if (!buf || len == 0) return -1;
+    const std::string &g_err = chromo::g_last_error;
     size_t n = g_err.size() < len - 1 ? g_err.size() : len - 1;
     memcpy(buf, g_err.data(), n);
     buf[n] = 0;
@@ -1594,52 +1473,25 @@ int chromo_density_init(const chromo_replica *d_reps, int32_t n_reps, void *stre
         if (r.field_on == 0) continue;
         if (r.B < 0 || r.B > CHROMO_MAX_BINDERS) return set_err(-1, "B out of range");
         int64_t nb = r.nx * r.ny * r.nz;
-        if (nb > 0x7fffffff || r.N > 0x7fffffff) return set_err(-1, "grid or chain too large");
+        if (nb > 2048LL * 2048LL || r.N > 0x7fffffff)
+            return set_err(-1, "grid or chain too large for the density scatter");
         maxN = r.N > maxN ? r.N : maxN;
         maxBins = nb > maxBins ? nb : maxBins;
     }
     if (maxN == 0) return 0;
-    int32_t *cell, *cnt, *off, *fill, *list, *bsum, *boff, *bsum2;
-    double *wts;
-    int64_t nblk = (maxBins + 2047) / 2048;
-    if (nblk > 2048) return set_err(-1, "grid too large for the init scan");
-    HIPC(hipMallocAsync((void **)&cell, sizeof(int32_t) * maxN, st));
-    HIPC(hipMallocAsync((void **)&list, sizeof(int32_t) * maxN, st));
-    HIPC(hipMallocAsync((void **)&wts, sizeof(double) * 3 * maxN, st));
-    HIPC(hipMallocAsync((void **)&cnt, sizeof(int32_t) * maxBins, st));
-    HIPC(hipMallocAsync((void **)&off, sizeof(int32_t) * maxBins, st));
-    HIPC(hipMallocAsync((void **)&fill, sizeof(int32_t) * maxBins, st));
-    HIPC(hipMallocAsync((void **)&bsum, sizeof(int32_t) * 2048, st));
-    HIPC(hipMallocAsync((void **)&boff, sizeof(int32_t) * 2048, st));
-    HIPC(hipMallocAsync((void **)&bsum2, sizeof(int32_t) * 2, st));
+    DensityScratch S;
+    HIPC(density_scratch_alloc(S, maxN, maxBins, st));
     for (int i = 0; i < n_reps; ++i) {
         const chromo_replica &r = h[i];
         if (r.field_on == 0) continue;
-        const chromo_replica *dr = d_reps + i;
-        int64_t nb = r.nx * r.ny * r.nz;
-        HIPC(hipMemsetAsync(cnt, 0, sizeof(int32_t) * nb, st));
-        HIPC(hipMemsetAsync(fill, 0, sizeof(int32_t) * nb, st));
-        unsigned gb = (unsigned)((r.N + 255) / 256);
-        unsigned gv = (unsigned)((nb + 255) / 256);
-        hipLaunchKernelGGL(k_init_cells, dim3(gb), dim3(256), 0, st, dr, cell, wts, cnt);
-        int64_t nbl = (nb + 2047) / 2048;
-        hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nbl), dim3(1024), 0, st, cnt, off, bsum, nb);
-        hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, bsum, boff, bsum2, nbl);
-        hipLaunchKernelGGL(k_scan_add, dim3(gv), dim3(256), 0, st, off, boff, nb);
-        hipLaunchKernelGGL(k_init_place, dim3(gb), dim3(256), 0, st, dr, cell, off, fill, list);
-        hipLaunchKernelGGL(k_init_sort, dim3(gv), dim3(256), 0, st, nb, off, cnt, list);
-        hipLaunchKernelGGL(k_init_gather, dim3(gv), dim3(256), 0, st, dr, off, cnt, list, wts, 1);
-        HIPC(launch_err());
+        PosView P{r.pos, r.N, 1};
+        hipError_t e = density_scatter(grid_of(r), r.N, r.B + 1, P, r.states, r.density, 1, S, st);
+        if (e != hipSuccess) {
+            density_scratch_free(S, st);
+            return set_err(-2, std::string("density scatter: ") + hipGetErrorString(e));
+        }
     }
-    HIPC(hipFreeAsync(cell, st));
-    HIPC(hipFreeAsync(list, st));
-    HIPC(hipFreeAsync(wts, st));
-    HIPC(hipFreeAsync(cnt, st));
-    HIPC(hipFreeAsync(off, st));
-    HIPC(hipFreeAsync(fill, st));
-    HIPC(hipFreeAsync(bsum, st));
-    HIPC(hipFreeAsync(boff, st));
-    HIPC(hipFreeAsync(bsum2, st));
+    density_scratch_free(S, st);
     return 0;
 }
 
@@ -1734,16 +1586,6 @@ int chromo_field_dE_tape(const chromo_replica *d_reps, int32_t rep, const int64_
                        g_lds_bp1);
     HIPC(launch_err());
     return 0;
-}
-
-int chromo_energy_reference(const chromo_replica *d_reps, int32_t n_reps, double *d_out,
-                            void *stream)
-{
-    (void)d_reps;
-    (void)n_reps;
-    (void)d_out;
-    (void)stream;
-    return set_err(-3, "chromo_energy_reference is not implemented yet");
 }
 
 }  // extern "C"

@@ -241,16 +241,14 @@ class PolymerBase(TransformedObject):
             df = pd.concat([vector_df, states_df, chem_df], axis=1)
         else:
             df = vector_df
-        df["bead_length"] = np.broadcast_to(
-            np.append(self.bead_length, 0).reshape((-1, 1)), (len(df.index), 1))
+        df["bead_length"] = np.append(self.bead_length, 0)[:len(df.index)]
         if "max_binders" in self._arrays:
-            df["max_binders"] = np.broadcast_to(self.max_binders, (len(df.index), 1))
+            df["max_binders"] = np.full(len(df.index), self.max_binders)
         for name in self._single_values:
             if hasattr(self, str(name)):
-                col = np.empty((len(df.index), 1), dtype=object)
-                col[0, 0] = str(getattr(self, str(name)))
+                col = np.full(len(df.index), "", dtype=object)
+                col[0] = str(getattr(self, str(name)))
                 df[name] = col
-                df[name] = df[name].apply(lambda x: x if x is not None else "")
         return df
 
     def to_csv(self, path: str):
@@ -297,7 +295,8 @@ class PolymerBase(TransformedObject):
     def from_file(cls, path, name=None, **kwargs):
         if name is None:
             name = path.split("/")[-1].split(".")[0]
-        df = pd.read_csv(path, header=[0, 1], index_col=0)
+        # round_trip parsing: snapshots restore coordinates bit for bit
+        df = pd.read_csv(path, header=[0, 1], index_col=0, float_precision="round_trip")
         return cls.from_dataframe(df, name, **kwargs)
 
 

@@ -196,11 +196,45 @@ int chromo_field_dE_tape(const chromo_replica *d_reps, int32_t rep, const int64_
 
 /* ---- absolute energies and adjoint -------------------------------------- */
 
+/* One chain in one field, for the energy entry points.  Device pointers:
+ * t3 (N x 3), states / marks (B x N), bonds (5 x (N-1)). */
+typedef struct {
+    int64_t N;
+    int32_t B;
+    int32_t pad_;
+    const double *t3;
+    const int8_t *states;
+    const int8_t *marks;
+    const double *bonds;
+    int64_t nx, ny, nz;
+    double x_width, y_width, z_width;
+    double bead_vol;
+    double vf_limit;
+    double mu_adjust;
+    double v_int[CHROMO_MAX_BINDERS];                        /* 4/3 pi r_int^3 */
+    int32_t cross_mask[CHROMO_MAX_BINDERS * CHROMO_MAX_BINDERS];  /* [i*4+j]: i lists j */
+    double F[CHROMO_MAX_BINDERS][CHROMO_MAX_MARK][CHROMO_MAX_SITES + 1];
+} chromo_energy_model;
+
+/* New (no reference counterpart): the energy whose differences are the move
+ * dE of SSWLC.compute_dE + UniformDensityField.compute_dE + binding_dE, and its
+ * gradient.  d_r: bead positions (N x 3).  theta = (chi, eps_ij for i <= j in
+ * row order, mu_j): 1 + B(B+1)/2 + B values.  d_E[4] = total, elastic, field,
+ * binding.  d_grad_r (N x 3) and d_grad_theta may be NULL.  Tangents fixed. */
+int chromo_energy_adjoint(const chromo_energy_model *model, const double *d_r,
+                          const double *d_theta, double *d_E, double *d_grad_r,
+                          double *d_grad_theta, void *stream);
+
 /* SSWLC.compute_E (chromo/polymers.pyx:1348-1381) and
- * UniformDensityField.compute_E (chromo/fields.pyx:1939-2286), per replica:
- * d_out[2*i] = elastic, d_out[2*i+1] = field.  Does not modify the density. */
-int chromo_energy_reference(const chromo_replica *d_reps, int32_t n_reps, double *d_out,
-                            void *stream);
+ * UniformDensityField.compute_E (chromo/fields.pyx:1939-2286): d_out[0] =
+ * elastic, d_out[1] = field (chi phi (1 - phi), round(phi, 2) > vf_limit, no
+ * cross-talk, as the reference).  d_pos holds r, t3, t2 (soa = 1: 9 x N rows;
+ * soa = 0: N x 9).  d_pref / d_intra: B field prefactors.  The recomputed
+ * densities (no 1e-18 clean-up, as update_all_densities) are written to
+ * d_density_out when it is not NULL. */
+int chromo_energy_reference(const chromo_energy_model *model, const double *d_pos, int32_t soa,
+                            const double *d_pref, const double *d_intra, double chi,
+                            double *d_density_out, double *d_out, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,93 @@
+"""The drop-in API (reference class and function names) on the GPU, checked
+against the oracle: a script written for chromo runs with an import change."""
+import copy
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _script_system(N=400, n_vox=8, seed=0):
+    from chromo_amd.binders import hp1, make_binder_collection, prc1
+    from chromo_amd.fields import UniformDensityField
+    from chromo_amd.polymers import Chromatin
+    rs = np.random.RandomState(seed)
+    np.random.seed(seed)
+    p = Chromatin.gaussian_walk_polymer(
+        "chr", N, np.full(N - 1, 25.0), states=np.zeros((N, 2), dtype=int),
+        binder_names=np.array(["HP1", "PRC1"]),
+        chemical_mods=(rs.random_sample((N, 2)) < 0.3).astype(int),
+        chemical_mod_names=np.array(["H3K9me3", "H3K27me3"]))
+    binders = make_binder_collection([hp1, prc1])
+    W = 30.0 * n_vox
+    udf = UniformDensityField([p], binders, W, n_vox, W, n_vox, W, n_vox)
+    return p, binders, udf
+
+
+def test_mc_sim_dropin_matches_oracle(oracle):
+    say("drop-in mc_sim")
+    from chromo_amd import mc
+    from chromo_amd.mc import mc_controller as ctl
+    from chromo_amd.mc.mc_sim import _ctrl_spec, seed_libc
+    p, binders, udf = _script_system()
+    bead_b, move_b = mc.get_amplitude_bounds([p])
+    ctrls = ctl.all_moves(".", bead_b.bounds, move_b.bounds, controller=ctl.SimpleControl)
+    chain0, field0 = p.chain_spec(), udf.field_spec()
+    specs0 = [_ctrl_spec(c) for c in ctrls]
+    # oracle: srand(1) (an unseeded libc) and numpy seeded like mc_sim
+    o = oracle.OracleSim(chain0, field0, seed=1)
+    assert np.array_equal(udf.density, o.density())
+    seed_libc(1)
+    for call, seed in enumerate([11, 12]):
+        mc.mc_sim([p], binders, 6, ctrls, udf, 1.0, seed)
+        o.np_seed(seed)
+        o.mc_sim(6, specs0)
+        r, t3, t2, st = o.state()
+        assert np.array_equal(p.states, st)
+        assert np.max(np.abs(p.r - r)) < 1e-9 and np.max(np.abs(p.t3 - t3)) < 1e-12
+        for c, s in zip(ctrls, specs0):
+            assert c.move.amp_bead == s.amp_bead and c.move.num_success == s.num_success
+            assert c.move.acceptance_tracker.acceptance_rate == s.rate
+            assert abs(c.move.amp_move - s.amp_move) <= 1e-15 * abs(s.amp_move)
+        one_bead = udf.n_bins / (udf.x_width * udf.y_width * udf.z_width)
+        assert np.max(np.abs(udf.density - o.density())) < 1e-9 * one_bead
+    # numpy's global stream continues from the device MT state (binding draws)
+    rs = np.random.RandomState(12)
+    k = sum(c.move.num_attempt for c in ctrls if c.move.name == "change_binding_state")
+    assert k > 0
+
+
+def test_polymer_in_field_snapshots_roundtrip(tmp_path):
+    say("polymer_in_field snapshots")
+    from chromo_amd import mc
+    from chromo_amd.polymers import Chromatin
+    p, binders, udf = _script_system(N=200, n_vox=6, seed=3)
+    bead_b, move_b = mc.get_amplitude_bounds([p])
+    mc.polymer_in_field([p], binders, udf, 3, 2, bead_b, move_b, random_seed=5,
+                        output_dir=str(tmp_path))
+    q = Chromatin.from_file(str(tmp_path / "chr-1.csv"))
+    assert np.allclose(q.r, p.r, rtol=0, atol=1e-12) and np.array_equal(q.states, p.states)
+    assert np.array_equal(q.chemical_mods, p.chemical_mods)
+
+
+def test_plain_launch_matches_persistent():
+    say("drop-in plain launch")
+    from chromo_amd import mc
+    from chromo_amd.mc import mc_controller as ctl
+    from chromo_amd.mc.mc_sim import seed_libc
+    res = []
+    for launch in ("persistent", "plain"):
+        p, binders, udf = _script_system(N=250, n_vox=6, seed=4)
+        bead_b, move_b = mc.get_amplitude_bounds([p])
+        ctrls = ctl.all_moves(".", bead_b.bounds, move_b.bounds, controller=ctl.SimpleControl)
+        seed_libc(1)
+        mc.mc_sim([p], binders, 3, ctrls, udf, 1.0, 9, launch=launch)
+        res.append((p.r.copy(), p.states.copy(), udf.density.copy()))
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(a, b)
