@@ -113,6 +113,7 @@ _SIGNATURES = {
     "chromo_np_seed": (ctypes.c_int, [_c_void_p, _c_i32, _c_void_p, _c_void_p]),
     "chromo_density_init": (ctypes.c_int, [_c_void_p, _c_i32, _c_void_p]),
     "chromo_mc_run": (ctypes.c_int, [_c_void_p, _c_i32, _c_i64, _c_void_p]),
+    "chromo_mc_run_tuned": (ctypes.c_int, [_c_void_p, _c_i32, _c_i64, _c_i32, _c_i32, _c_void_p]),
     "chromo_mc_run_plain": (ctypes.c_int, [_c_void_p, _c_i32, _c_i64, _c_void_p]),
     "chromo_mc_step": (ctypes.c_int, [_c_void_p, _c_i32, _c_i32, _c_void_p]),
     "chromo_update_amplitudes": (ctypes.c_int, [_c_void_p, _c_i32, _c_i32, _c_void_p]),

@@ -2,7 +2,14 @@
 
     python -m chromo_amd.build
 
-compiles chromo_amd/csrc/chromo_hip.hip and chromo_energy.hip with hipcc into chromo_amd/libchromo_hip.so.
+Compiles the translation units below with hipcc, in parallel, and links them into
+chromo_amd/libchromo_hip.so:
+
+* chromo_hip.hip: one-wavefront kernels, RNG, density, tape, host entry points;
+* chromo_energy.hip: energies and their adjoint;
+* chromo_spec.hip, once per W in (2, 4, 8): the speculative kernel with W
+  wavefronts per replica.
+
 -ffp-contract=off keeps every multiply and add separately rounded, as in the
 reference's x86-64 build; packed FP32 instructions are disabled.
 """
@@ -12,18 +19,26 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 ROOT = HERE.parent
-SRCS = [HERE / "csrc" / "chromo_hip.hip", HERE / "csrc" / "chromo_energy.hip"]
+CSRC = HERE / "csrc"
 OUT = HERE / "libchromo_hip.so"
 ARCH = os.environ.get("CHROMO_OFFLOAD_ARCH", "gfx950")
+SPEC_WAVES = (2, 4)
+
+# (source, extra flags, object name)
+UNITS = [(CSRC / "chromo_hip.hip", [], "chromo_hip.o"),
+         (CSRC / "chromo_energy.hip", [], "chromo_energy.o")] + \
+        [(CSRC / "chromo_spec.hip", [f"-DCHROMO_SPEC_W={w}"], f"chromo_spec_w{w}.o") for w in SPEC_WAVES]
 
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-fPIC", "-shared", f"-I{ROOT / 'include'}", f"-I{HERE / 'csrc'}",
+    "-fPIC", f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]
 
 
@@ -34,16 +49,28 @@ def hipcc() -> str:
     return p
 
 
-def build(verbose: bool = False) -> Path:
-    tmp = OUT.with_suffix(".so.tmp")
-    cmd = [hipcc(), *FLAGS, "-o", str(tmp), *[str(s) for s in SRCS]]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError(f"hipcc failed ({r.returncode})")
-    os.replace(tmp, OUT)
+        raise RuntimeError(f"hipcc failed ({r.returncode}): {' '.join(cmd[-3:])}")
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> Path:
+    cc = hipcc()
+    jobs = jobs or min(len(UNITS), max(1, int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with tempfile.TemporaryDirectory(prefix="chromo_build_") as td:
+        objs = [str(Path(td) / o) for _, _, o in UNITS]
+        cmds = [[cc, *FLAGS, *extra, "-c", str(src), "-o", obj]
+                for (src, extra, _), obj in zip(UNITS, objs)]
+        with ThreadPoolExecutor(max_workers=jobs) as ex:
+            for f in [ex.submit(_run, c, verbose) for c in cmds]:
+                f.result()
+        tmp = OUT.with_suffix(".so.tmp")
+        _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs], verbose)
+        os.replace(tmp, OUT)
     return OUT
 
 

@@ -39,7 +39,25 @@ enum : int {
     ERR_NBINDERS = 4,     // binding move on a chain without binders
 };
 
-__device__ __forceinline__ int lane_id() { return (int)threadIdx.x; }
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
+// Wavefront index in the workgroup, as a wave-uniform (scalar) value.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE)); }
+
+// Orders this wavefront's LDS accesses before / after the call for all of its
+// lanes (the LDS executes one wavefront's operations in order).  Used by code
+// that one wavefront of a workgroup runs on its own.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// As wave_sync, and also waits for this wavefront's global-memory accesses.
+__device__ __forceinline__ void wave_sync_global()
+{
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
 
 // ---------------------------------------------------------------------------
 // Wave-level helpers
@@ -161,7 +179,7 @@ __device__ __forceinline__ uint32_t philox_draw(WaveRng &R)
 // MT19937 regeneration of the 624-word block (lane 0, LDS), then tempering.
 __device__ void mt_twist(uint32_t *mt)
 {
-    __syncthreads();
+    wave_sync();
     if (lane_id() == 0) {
         for (int i = 0; i < 624; ++i) {
             uint32_t y = (mt[i] & 0x80000000u) | (mt[i + 1 == 624 ? 0 : i + 1] & 0x7fffffffu);
@@ -172,7 +190,7 @@ __device__ void mt_twist(uint32_t *mt)
             mt[i] = v;
         }
     }
-    __syncthreads();
+    wave_sync();
 }
 
 __device__ __forceinline__ uint32_t mt_draw(WaveRng &R)
@@ -219,14 +237,21 @@ __device__ __forceinline__ int64_t rnd_randint(WaveRng &R, int64_t high)
 // Geometry (chromo/util/linalg.pyx, chromo/mc/move_funcs.pyx)
 // ---------------------------------------------------------------------------
 
-// uniform_sample_unit_sphere, linalg.pyx:23-59
-__device__ __forceinline__ void sample_unit_sphere(WaveRng &R, double v[3])
+// uniform_sample_unit_sphere, linalg.pyx:23-59, from its two rand() draws
+__device__ __forceinline__ void sphere_from(int32_t d1, int32_t d2, double v[3])
 {
-    double phi = (double)rnd31(R) / RMAX * (2.0 * M_PI);
-    double theta = acos(((double)rnd31(R) / RMAX) * 2 - 1);
+    double phi = (double)d1 / RMAX * (2.0 * M_PI);
+    double theta = acos(((double)d2 / RMAX) * 2 - 1);
     v[0] = cos(phi) * sin(theta);
     v[1] = sin(phi) * sin(theta);
     v[2] = cos(theta);
+}
+
+__device__ __forceinline__ void sample_unit_sphere(WaveRng &R, double v[3])
+{
+    int32_t d1 = rnd31(R);
+    int32_t d2 = rnd31(R);
+    sphere_from(d1, d2, v);
 }
 
 // arbitrary_axis_rotation, linalg.pyx:62-139 (rows 0..2 of the 4x4 matrix)
@@ -280,7 +305,7 @@ __device__ __forceinline__ int64_t imax64(int64_t a, int64_t b) { return a > b ?
 __device__ __forceinline__ int64_t imin64(int64_t a, int64_t b) { return a < b ? a : b; }
 
 // capped_exponential, bead_selection.pyx:19-66
-__device__ int64_t capped_exponential(WaveRng &R, int64_t window, int64_t cap)
+__device__ __forceinline__ int64_t capped_exponential(WaveRng &R, int64_t window, int64_t cap)
 {
     int64_t r;
     do {
@@ -305,7 +330,7 @@ __device__ __forceinline__ int64_t from_right(WaveRng &R, int64_t w, int64_t nb,
     return nb - d;
 }
 
-__device__ int64_t from_point(WaveRng &R, int64_t w, int64_t nb, int64_t ind0, int &err)
+__device__ __forceinline__ int64_t from_point(WaveRng &R, int64_t w, int64_t nb, int64_t ind0, int &err)
 {
     if (w > nb) {
         err = ERR_WINDOW;
@@ -381,7 +406,7 @@ __device__ __forceinline__ double E_pair(const Bond &bd, const double *bend, dou
 }
 
 // bead_pair_dE_poly_forward, polymers.pyx:1177-1277
-__device__ double pair_dE_forward(const Bond &bd, const double *r0, const double *r1,
+__device__ __forceinline__ double pair_dE_forward(const Bond &bd, const double *r0, const double *r1,
                                   const double *tr1, const double *t30, const double *t31,
                                   const double *tt31)
 {
@@ -401,7 +426,7 @@ __device__ double pair_dE_forward(const Bond &bd, const double *r0, const double
 }
 
 // bead_pair_dE_poly_reverse, polymers.pyx:1279-1346
-__device__ double pair_dE_reverse(const Bond &bd, const double *r0, const double *tr0,
+__device__ __forceinline__ double pair_dE_reverse(const Bond &bd, const double *r0, const double *tr0,
                                   const double *r1, const double *t30, const double *tt30,
                                   const double *t31)
 {
@@ -423,7 +448,7 @@ __device__ double pair_dE_reverse(const Bond &bd, const double *r0, const double
 // bead_binding_dE, polymers.pyx:1408-1538, with host-built -log(single-site Z)
 // tables F[b][marks][state].
 template <int NB>
-__device__ double bead_binding_dE(const chromo_replica *R, const int8_t *mods, const int8_t *st,
+__device__ __forceinline__ double bead_binding_dE(const chromo_replica *R, const int8_t *mods, const int8_t *st,
                                   const int8_t *stt)
 {
     constexpr int B = NB;
@@ -449,6 +474,69 @@ __device__ double bead_binding_dE(const chromo_replica *R, const int8_t *mods, c
             dE += (double)st[b] * mu * f;
         }
     }
+    return dE;
+}
+
+// ---------------------------------------------------------------------------
+// Field dE terms (chromo/fields.pyx:1675-1842)
+// ---------------------------------------------------------------------------
+
+// One voxel's share of get_dE_binders_and_beads (fields.pyx:1723-1750) and of
+// nonspecific_interact_dE (fields.pyx:1792-1842), added to a lane's running sums.
+// r: current densities, a: delta densities (B+1 species each).
+template <int NB>
+__device__ __forceinline__ void voxel_dE_terms(const chromo_replica *R, const double *r, const double *a,
+                                               double *S2, double *SX, double &NS)
+{
+    for (int j = 0; j < NB; ++j) {
+        double aj = r[j + 1] + a[j + 1];
+        double d2 = aj * aj - r[j + 1] * r[j + 1];
+        if (fabs(d2) < 1E-18) d2 = 0;
+        S2[j] += d2;
+        for (int kk = 0; kk < NB; ++kk) {
+            double dx = (aj * (r[kk + 1] + a[kk + 1])) - (r[j + 1] * r[kk + 1]);
+            if (fabs(dx) < 1E-18) dx = 0;
+            SX[j * NB + kk] += dx;
+        }
+    }
+    const double V = R->bead_vol, vf = R->vf_limit, chi = R->chi, vol_bin = R->vol_bin;
+    double vf0 = r[0] * V;
+    double vf1 = vf0 + (a[0] * V);
+    if (vf1 > vf) NS += E_HUGE_FIELD * vf1;
+    else NS += chi * (vol_bin / V) * (vf1 * vf1);
+    if (vf0 > vf) NS -= E_HUGE_FIELD * vf0;
+    else NS -= chi * (vol_bin / V) * (vf0 * vf0);
+}
+
+// A lane's running sums weighted by the binder prefactors and cross-talk
+// coefficients; summed over the wavefront this is the binder-pair, cross-talk and
+// chi part of the field dE.
+template <int NB>
+__device__ __forceinline__ double lane_field_share(const chromo_replica *R, const double *S2,
+                                                   const double *SX, double NS)
+{
+    double p = 0;
+    for (int i = 0; i < NB; ++i) p += R->pref[i] * S2[i];
+    for (int i = 0; i < NB; ++i)
+        for (int j = 0; j < NB; ++j) p += R->cross[i * NB + j] * SX[i * NB + j];
+    p += NS;
+    return p;
+}
+
+// UniformDensityField.compute_dE's total (fields.pyx:1209-1233): confinement
+// (trial minus current), the wave-summed share, the doubly-bound terms.
+template <int NB>
+__device__ __forceinline__ double field_dE_total(const chromo_replica *R, double share, const int *dbl_t,
+                                                 const int *dbl, int out_t, int out_c, bool state_change)
+{
+    double dE = 0;
+    if (!state_change) {
+        dE += (double)out_t * E_HUGE_FIELD;
+        dE -= (double)out_c * E_HUGE_FIELD;
+    }
+    double dEb = share;
+    for (int i = 0; i < NB; ++i) dEb += R->intra[i] * (double)(dbl_t[i] - dbl[i]);
+    dE += dEb;
     return dE;
 }
 

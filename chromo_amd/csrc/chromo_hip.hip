@@ -1,13 +1,7 @@
-// chromo_hip.hip -- MI355X (gfx950) kernels and C ABI for chromo's Monte Carlo inner loop.
-//
-// One wavefront per replica runs that replica's sequential Markov chain
-// (chromo/mc/mc_sim.pyx:26-182).  A move's affected beads are spread over the
-// 64 lanes; its 16*n voxel contributions (2 configurations x 8 corners per bead)
-// are grouped per voxel in an arena (LDS for n <= SEG_LDS, per-replica global
-// scratch above that) through a hash table, ordered by contribution index and
-// folded per voxel in the reference's order, so each voxel's delta density is
-// the reference's left fold (chromo/fields.pyx:1491-1520) bit for bit.  Energy
-// terms are reduced with wavefront shuffles; no global atomics are used.
+// chromo_hip.hip -- MI355X (gfx950) kernels and C ABI for chromo's Monte Carlo inner loop:
+// the one-wavefront persistent and one-move-per-launch kernels, RNG seeding,
+// density initialisation, the field-dE tape and the host entry points.  The
+// speculative multi-wavefront kernel is in chromo_spec.hip.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -18,950 +12,17 @@
 #include <string>
 #include <vector>
 
-#include "chromo_device.hpp"
 #include "chromo_density.hpp"
+#include "chromo_mc.hpp"
 
 using namespace chromo;
 
+namespace chromo {
+hipError_t spec_launch_w2(const chromo_replica *, int, int64_t, int, hipStream_t);
+hipError_t spec_launch_w4(const chromo_replica *, int, int64_t, int, hipStream_t);
+}  // namespace chromo
+
 namespace {
-
-// ---------------------------------------------------------------------------
-// Move arena
-// ---------------------------------------------------------------------------
-
-struct ArenaLayout {
-    size_t bnew, tanT, epart, pw, udrho, urho;     // double arrays
-    size_t inds;                                   // int64
-    size_t evox, eslot, list, hkey, hmin, hu, uvox, uslot, ucnt, uoff;   // int32
-    size_t stold, stnew;                           // int8
-    size_t total;
-    int cap, ecap, H, hbits;
-};
-
-__host__ __device__ inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
-
-// cap = per-bead capacity (all moves), fcap = field-move capacity (<= cap): the
-// per-contribution arrays hold 16 * fcap entries.
-__host__ __device__ inline ArenaLayout arena_layout(int cap, int fcap, int Bp1, int H)
-{
-    ArenaLayout L;
-    L.cap = cap;
-    L.ecap = 16 * fcap;
-    L.H = H;
-    int hb = 0;
-    while ((1 << hb) < H) ++hb;
-    L.hbits = hb;
-    size_t o = 0;
-    L.bnew = o;  o += sizeof(double) * 9 * (size_t)cap;
-    L.tanT = o;  o += sizeof(double) * 9 * (size_t)cap;
-    L.epart = o; o += sizeof(double) * (size_t)cap;
-    L.pw = o;    o += sizeof(double) * 3 * 2 * (size_t)cap;
-    L.udrho = o; o += sizeof(double) * (size_t)Bp1 * L.ecap;
-    L.urho = o;  o += sizeof(double) * (size_t)Bp1 * L.ecap;
-    L.inds = o;  o += sizeof(int64_t) * (size_t)cap;
-    L.evox = o;  o += sizeof(int32_t) * (size_t)L.ecap;
-    L.eslot = o; o += sizeof(int32_t) * (size_t)L.ecap;
-    L.list = o;  o += sizeof(int32_t) * (size_t)L.ecap;
-    L.hkey = o;  o += sizeof(int32_t) * (size_t)H;
-    L.hmin = o;  o += sizeof(int32_t) * (size_t)H;
-    L.hu = o;    o += sizeof(int32_t) * (size_t)H;
-    L.uvox = o;  o += sizeof(int32_t) * (size_t)L.ecap;
-    L.uslot = o; o += sizeof(int32_t) * (size_t)L.ecap;
-    L.ucnt = o;  o += sizeof(int32_t) * (size_t)L.ecap;
-    L.uoff = o;  o += sizeof(int32_t) * (size_t)(L.ecap + 1);
-    L.stold = o; o += (size_t)CHROMO_MAX_BINDERS * cap;
-    L.stnew = o; o += (size_t)CHROMO_MAX_BINDERS * cap;
-    L.total = align_up(o, 256);
-    return L;
-}
-
-__host__ __device__ inline int hash_size_for(int fcap)
-{
-    int h = 1;
-    while (h < 2 * 16 * fcap) h <<= 1;
-    return h;
-}
-
-__host__ __device__ inline ArenaLayout lds_layout(int Bp1)
-{
-    return arena_layout(SEG_LDS, SEG_LDS, Bp1, HASH_LDS);
-}
-
-__host__ __device__ inline ArenaLayout global_layout(int64_t bead_cap, int64_t seg_cap, int Bp1)
-{
-    int fc = seg_cap > 0 ? (int)seg_cap : 1;
-    return arena_layout((int)bead_cap, fc, Bp1, hash_size_for(fc));
-}
-
-struct Arena {
-    double *bnew, *tanT, *epart, *pw, *udrho, *urho;
-    int64_t *inds;
-    int32_t *evox, *eslot, *list, *hkey, *hmin, *hu, *uvox, *uslot, *ucnt, *uoff;
-    int8_t *stold, *stnew;
-    int cap, ecap, hmask, hshift;
-};
-
-__device__ inline Arena make_arena(char *base, const ArenaLayout &L)
-{
-    Arena A;
-    A.bnew = (double *)(base + L.bnew);
-    A.tanT = (double *)(base + L.tanT);
-    A.epart = (double *)(base + L.epart);
-    A.pw = (double *)(base + L.pw);
-    A.udrho = (double *)(base + L.udrho);
-    A.urho = (double *)(base + L.urho);
-    A.inds = (int64_t *)(base + L.inds);
-    A.evox = (int32_t *)(base + L.evox);
-    A.eslot = (int32_t *)(base + L.eslot);
-    A.list = (int32_t *)(base + L.list);
-    A.hkey = (int32_t *)(base + L.hkey);
-    A.hmin = (int32_t *)(base + L.hmin);
-    A.hu = (int32_t *)(base + L.hu);
-    A.uvox = (int32_t *)(base + L.uvox);
-    A.uslot = (int32_t *)(base + L.uslot);
-    A.ucnt = (int32_t *)(base + L.ucnt);
-    A.uoff = (int32_t *)(base + L.uoff);
-    A.stold = (int8_t *)(base + L.stold);
-    A.stnew = (int8_t *)(base + L.stnew);
-    A.cap = L.cap;
-    A.ecap = L.ecap;
-    A.hmask = L.H - 1;
-    A.hshift = 32 - L.hbits;
-    return A;
-}
-
-constexpr size_t MT_LDS_BYTES = 624 * sizeof(uint32_t);
-
-inline size_t lds_bytes_for(int Bp1)
-{
-    ArenaLayout L = lds_layout(Bp1);
-    return L.total + MT_LDS_BYTES;
-}
-
-extern __shared__ char lds_dyn[];
-
-// ---------------------------------------------------------------------------
-// Per-move result
-// ---------------------------------------------------------------------------
-
-struct MoveOut {
-    int64_t n, ind0, indf, nbins;
-    double dEp, dEf, u, expdE;
-    int accepted;
-    int err;
-};
-
-// ---------------------------------------------------------------------------
-// Move body: everything after the bead range is known.  KIND 0 = LDS arena,
-// KIND 1 = global scratch arena.
-// ---------------------------------------------------------------------------
-
-struct Proposal {
-    int move;
-    int64_t n, ind0, indf;
-    double T[12];          // crank / end pivot rotation, slide translation in T[3,7,11]
-    int binder;
-    // per-lane registers for beads i == lane (< 64)
-    int64_t mypick;        // tangent bead index
-    double myT[9];         // tangent rotation
-    int mynew;             // binding new state
-};
-
-template <int KIND, int NB>
-__device__ void move_body(const chromo_replica *R, WaveRng &rng, Proposal &P, MoveOut &out)
-{
-    const int lane = lane_id();
-    const int64_t N = R->N;
-    constexpr int B = NB;
-    constexpr int Bp1 = NB + 1;
-    const int mv = P.move;
-    const int64_t n = P.n;
-    const int64_t ind0 = P.ind0, indf = P.indf;
-    const bool check_field = R->field_on && mv != CHROMO_TANGENT_ROTATION;
-    const bool state_change = (mv == CHROMO_CHANGE_BINDING_STATE);
-
-    Arena A;
-    if (KIND == 0) {
-        A = make_arena(lds_dyn, lds_layout(Bp1));
-    } else {
-        A = make_arena((char *)R->scratch, global_layout(R->bead_cap, R->seg_cap, Bp1));
-    }
-    const int cap = A.cap;
-    double *Pp = R->pos;
-
-    // ---- per-bead: old and trial coordinates, states, elastic/binding partials,
-    //      voxel corners of the current (k = 0) and trial (k = 1) positions.
-    for (int64_t i = lane; i < n; i += WAVE) {
-        int64_t b;
-        if (mv == CHROMO_TANGENT_ROTATION) b = (i < WAVE) ? P.mypick : A.inds[i];
-        else b = ind0 + i;
-        double o[9], w[9];
-#pragma unroll
-        for (int c = 0; c < 9; ++c) o[c] = Pp[c * N + b];
-        if (mv == CHROMO_CRANK_SHAFT || mv == CHROMO_END_PIVOT) {
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                w[j] = rowdot(&P.T[4 * j], o[0], o[1], o[2]) + P.T[4 * j + 3];
-                w[3 + j] = rowdot(&P.T[4 * j], o[3], o[4], o[5]);
-                w[6 + j] = rowdot(&P.T[4 * j], o[6], o[7], o[8]);
-            }
-        } else if (mv == CHROMO_SLIDE) {
-#pragma unroll
-            for (int j = 0; j < 3; ++j) w[j] = o[j] + P.T[4 * j + 3];
-#pragma unroll
-            for (int c = 3; c < 9; ++c) w[c] = o[c];
-        } else if (mv == CHROMO_TANGENT_ROTATION) {
-            double Tl[9];
-            if (i < WAVE) {
-#pragma unroll
-                for (int q = 0; q < 9; ++q) Tl[q] = P.myT[q];
-            } else {
-#pragma unroll
-                for (int q = 0; q < 9; ++q) Tl[q] = A.tanT[q * cap + i];
-            }
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                w[j] = o[j];
-                w[3 + j] = rowdot(&Tl[3 * j], o[3], o[4], o[5]);
-                w[6 + j] = rowdot(&Tl[3 * j], o[6], o[7], o[8]);
-            }
-        } else {
-#pragma unroll
-            for (int c = 0; c < 9; ++c) w[c] = o[c];
-        }
-#pragma unroll
-        for (int c = 0; c < 9; ++c) A.bnew[c * cap + i] = w[c];
-        int8_t so[CHROMO_MAX_BINDERS], sn[CHROMO_MAX_BINDERS];
-        for (int m = 0; m < B; ++m) {
-            so[m] = R->states[(int64_t)m * N + b];
-            sn[m] = so[m];
-        }
-        if (state_change) {
-            int v = (i < WAVE) ? P.mynew : (int)A.stnew[P.binder * cap + i];
-            sn[P.binder] = (int8_t)v;
-        }
-        for (int m = 0; m < B; ++m) {
-            A.stold[m * cap + i] = so[m];
-            A.stnew[m * cap + i] = sn[m];
-        }
-        // elastic / binding partials
-        if (mv == CHROMO_TANGENT_ROTATION) {
-            double e = 0;
-            if (b != 0) {
-                double r0[3], t30[3];
-                for (int j = 0; j < 3; ++j) {
-                    r0[j] = Pp[j * N + b - 1];
-                    t30[j] = Pp[(3 + j) * N + b - 1];
-                }
-                Bond bd = load_bond(R->bonds, N, b - 1);
-                e += pair_dE_forward(bd, r0, &o[0], &w[0], t30, &o[3], &w[3]);
-            }
-            if (b + 1 != N) {
-                double r1[3], t31[3];
-                for (int j = 0; j < 3; ++j) {
-                    r1[j] = Pp[j * N + b + 1];
-                    t31[j] = Pp[(3 + j) * N + b + 1];
-                }
-                Bond bd = load_bond(R->bonds, N, b);
-                e += pair_dE_reverse(bd, &o[0], &w[0], r1, &o[3], &w[3], t31);
-            }
-            A.epart[i] = e;
-        } else if (mv == CHROMO_CHANGE_BINDING_STATE) {
-            int8_t mods[CHROMO_MAX_BINDERS];
-            for (int m = 0; m < B; ++m) mods[m] = R->marks[(int64_t)m * N + b];
-            A.epart[i] = bead_binding_dE<NB>(R, mods, so, sn);
-        } else {
-            if (i == 0 && ind0 != 0) {
-                double r0[3], t30[3];
-                for (int j = 0; j < 3; ++j) {
-                    r0[j] = Pp[j * N + ind0 - 1];
-                    t30[j] = Pp[(3 + j) * N + ind0 - 1];
-                }
-                Bond bd = load_bond(R->bonds, N, ind0 - 1);
-                A.epart[0] = pair_dE_forward(bd, r0, &o[0], &w[0], t30, &o[3], &w[3]);
-            }
-            if (i == n - 1 && indf != N) {
-                double r1[3], t31[3];
-                for (int j = 0; j < 3; ++j) {
-                    r1[j] = Pp[j * N + indf];
-                    t31[j] = Pp[(3 + j) * N + indf];
-                }
-                Bond bd = load_bond(R->bonds, N, indf - 1);
-                A.epart[1] = pair_dE_reverse(bd, &o[0], &w[0], r1, &o[3], &w[3], t31);
-            }
-        }
-        if (check_field) {
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const double *pos = (k == 0 || state_change) ? &o[0] : &w[0];
-                int64_t ix[3];
-                double wt[3];
-                voxel_lower(R, pos, ix, wt);
-                const int p = (int)(2 * i + k);
-                for (int j = 0; j < 3; ++j) A.pw[j * 2 * cap + p] = wt[j];
-#pragma unroll
-                for (int l = 0; l < 8; ++l) A.evox[p * 8 + l] = (int32_t)corner_voxel(R, ix, l);
-            }
-        }
-    }
-    __syncthreads();
-
-    // ---- polymer dE (SSWLC.compute_dE, polymers.pyx:1024-1082), uniform
-    double dEp = 0;
-    if (mv == CHROMO_TANGENT_ROTATION || mv == CHROMO_CHANGE_BINDING_STATE) {
-        for (int64_t i = 0; i < n; ++i) dEp += A.epart[i];
-    } else {
-        if (ind0 != 0) dEp += A.epart[0];
-        if (indf != N) dEp += A.epart[1];
-    }
-
-    // ---- field dE (UniformDensityField.compute_dE, fields.pyx:1149-1233)
-    double dEf = 0;
-    int U = 0;
-    if (check_field) {
-        const int E = (int)(16 * n);
-        // unique voxels: hash insert, first-occurrence index per voxel
-        for (int e = lane; e < E; e += WAVE) {
-            int v = A.evox[e];
-            uint32_t h = ((uint32_t)v * 2654435761u) >> A.hshift;
-            for (;;) {
-                int old = atomicCAS(&A.hkey[h], HKEY_EMPTY, v);
-                if (old == HKEY_EMPTY || old == v) break;
-                h = (h + 1) & (uint32_t)A.hmask;
-            }
-            atomicMin(&A.hmin[h], e);
-            A.eslot[e] = (int)h;
-        }
-        __syncthreads();
-        // number unique voxels in first-occurrence order
-        const int C = (E + WAVE - 1) / WAVE;
-        const int e0 = lane * C;
-        const int e1 = min(E, e0 + C);
-        int cnt = 0;
-        for (int e = e0; e < e1; ++e)
-            if (A.hmin[A.eslot[e]] == e) ++cnt;
-        int base = wave_excl_scan(cnt, U);
-        for (int e = e0; e < e1; ++e) {
-            int s = A.eslot[e];
-            if (A.hmin[s] == e) {
-                A.uvox[base] = A.evox[e];
-                A.uslot[base] = s;
-                A.hu[s] = base;
-                A.ucnt[base] = 0;
-                ++base;
-            }
-        }
-        __syncthreads();
-        // gather current densities of the touched voxels
-        const double *rho = R->density;
-        for (int q = lane; q < U; q += WAVE) {
-            int64_t v = A.uvox[q];
-            for (int m = 0; m < Bp1; ++m) A.urho[m * A.ecap + q] = rho[v * Bp1 + m];
-        }
-        for (int e = lane; e < E; e += WAVE) atomicAdd(&A.ucnt[A.hu[A.eslot[e]]], 1);
-        __syncthreads();
-        {
-            const int Cu = (U + WAVE - 1) / WAVE;
-            const int q0 = lane * Cu, q1 = min(U, q0 + Cu);
-            int s = 0;
-            for (int q = q0; q < q1; ++q) s += A.ucnt[q];
-            int tot;
-            int off = wave_excl_scan(s, tot);
-            for (int q = q0; q < q1; ++q) {
-                A.uoff[q] = off;
-                off += A.ucnt[q];
-            }
-            if (lane == 0) A.uoff[U] = E;
-        }
-        __syncthreads();
-        for (int e = lane; e < E; e += WAVE) {
-            int q = A.hu[A.eslot[e]];
-            int pos = A.uoff[q] + atomicSub(&A.ucnt[q], 1) - 1;
-            A.list[pos] = e;
-        }
-        __syncthreads();
-        // per voxel: order contributions, fold (fields.pyx:1491-1520), energy terms
-        double S2[CHROMO_MAX_BINDERS], SX[CHROMO_MAX_BINDERS * CHROMO_MAX_BINDERS], NS = 0;
-        for (int j = 0; j < CHROMO_MAX_BINDERS; ++j) S2[j] = 0;
-        for (int j = 0; j < CHROMO_MAX_BINDERS * CHROMO_MAX_BINDERS; ++j) SX[j] = 0;
-        const double vol_bin = R->vol_bin, V = R->bead_vol, vf = R->vf_limit, chi = R->chi;
-        for (int q = lane; q < U; q += WAVE) {
-            const int s0 = A.uoff[q], s1 = A.uoff[q + 1];
-            for (int a = s0 + 1; a < s1; ++a) {
-                int x = A.list[a];
-                int c = a - 1;
-                while (c >= s0 && A.list[c] > x) {
-                    A.list[c + 1] = A.list[c];
-                    --c;
-                }
-                A.list[c + 1] = x;
-            }
-            double acc[CHROMO_MAX_BINDERS + 1];
-            for (int m = 0; m < Bp1; ++m) acc[m] = 0;
-            for (int a = s0; a < s1; ++a) {
-                const int e = A.list[a];
-                const int p = e >> 3, l = e & 7;
-                const int i = p >> 1, k = p & 1;
-                const double wt = corner_weight(l, A.pw[p], A.pw[2 * cap + p], A.pw[4 * cap + p]);
-                const double c0 = wt / vol_bin;
-                const double pre = k ? 1. : -1.;
-                const double t0 = pre * c0;
-                if (fabs(t0) > 1E-18) acc[0] += t0;
-                for (int m = 1; m < Bp1; ++m) {
-                    int st = (k == 0 || !state_change) ? A.stold[(m - 1) * cap + i]
-                                                       : A.stnew[(m - 1) * cap + i];
-                    double t = pre * (c0 * (double)st);
-                    if (fabs(t) > 1E-18) acc[m] += t;
-                }
-            }
-            double r[CHROMO_MAX_BINDERS + 1];
-            for (int m = 0; m < Bp1; ++m) {
-                A.udrho[m * A.ecap + q] = acc[m];
-                r[m] = A.urho[m * A.ecap + q];
-            }
-            for (int j = 0; j < B; ++j) {
-                double aj = r[j + 1] + acc[j + 1];
-                double d2 = aj * aj - r[j + 1] * r[j + 1];
-                if (fabs(d2) < 1E-18) d2 = 0;
-                S2[j] += d2;
-                for (int kk = 0; kk < B; ++kk) {
-                    double dx = (aj * (r[kk + 1] + acc[kk + 1])) - (r[j + 1] * r[kk + 1]);
-                    if (fabs(dx) < 1E-18) dx = 0;
-                    SX[j * B + kk] += dx;
-                }
-            }
-            double vf0 = r[0] * V;
-            double vf1 = vf0 + (acc[0] * V);
-            if (vf1 > vf) NS += E_HUGE_FIELD * vf1;
-            else NS += chi * (vol_bin / V) * (vf1 * vf1);
-            if (vf0 > vf) NS -= E_HUGE_FIELD * vf0;
-            else NS -= chi * (vol_bin / V) * (vf0 * vf0);
-        }
-        for (int j = 0; j < B; ++j) S2[j] = wave_sum(S2[j]);
-        for (int j = 0; j < B * B; ++j) SX[j] = wave_sum(SX[j]);
-        NS = wave_sum(NS);
-        // count_doubly_bound (fields.pyx:1877-1937) and confinement (:121-202)
-        int dbl_t[CHROMO_MAX_BINDERS], dbl[CHROMO_MAX_BINDERS];
-        for (int j = 0; j < B; ++j) {
-            dbl_t[j] = 0;
-            dbl[j] = 0;
-        }
-        int out_t = 0, out_c = 0;
-        for (int64_t i = lane; i < n; i += WAVE) {
-            for (int j = 0; j < B; ++j) {
-                int so = A.stold[j * cap + i];
-                int stt = state_change ? A.stnew[j * cap + i] : so;
-                dbl_t[j] += (stt == 2);
-                dbl[j] += (so == 2);
-            }
-            if (R->confine_type != 0 && !state_change) {
-                int64_t b = ind0 + i;
-                double rc[3], rt[3];
-                for (int j = 0; j < 3; ++j) {
-                    rc[j] = Pp[j * N + b];
-                    rt[j] = A.bnew[j * cap + i];
-                }
-                if (R->confine_type == 1) {
-                    out_t += sqrt(dot3(rt, rt)) > R->confine_length;
-                    out_c += sqrt(dot3(rc, rc)) > R->confine_length;
-                } else {
-                    for (int j = 0; j < 3; ++j) out_t += fabs(rt[j]) > R->confine_length / 2;
-                }
-            }
-        }
-        for (int j = 0; j < B; ++j) {
-            dbl_t[j] = bcast0(wave_sum_int(dbl_t[j]));
-            dbl[j] = bcast0(wave_sum_int(dbl[j]));
-        }
-        out_t = bcast0(wave_sum_int(out_t));
-        out_c = bcast0(wave_sum_int(out_c));
-        double dE = 0;
-        if (!state_change) {
-            dE += (double)out_t * E_HUGE_FIELD;
-            dE -= (double)out_c * E_HUGE_FIELD;
-        }
-        double dEb = 0.0;
-        for (int i = 0; i < B; ++i) {
-            dEb += R->pref[i] * S2[i];
-            dEb += R->intra[i] * (double)(dbl_t[i] - dbl[i]);
-        }
-        for (int i = 0; i < B; ++i)
-            for (int j = 0; j < B; ++j) dEb += R->cross[i * B + j] * SX[i * B + j];
-        dEb += NS;
-        dE += dEb;
-        dEf = dE;
-    }
-
-    // ---- Metropolis (mc_sim.pyx:152-182)
-    double dE = 0;
-    dE += dEp;
-    if (check_field) dE += dEf;
-    double expdE = exp(-dE);
-    double u = rnd_unit(rng);
-    int acc = u < expdE;
-
-    // ---- commit (MCAdapter.accept, moves.pyx:190-226; update_affected_densities,
-    //      fields.pyx:1968-1975)
-    if (acc) {
-        for (int64_t i = lane; i < n; i += WAVE) {
-            int64_t b;
-            if (mv == CHROMO_TANGENT_ROTATION) b = (i < WAVE) ? P.mypick : A.inds[i];
-            else b = ind0 + i;
-            if (mv == CHROMO_CHANGE_BINDING_STATE) {
-                R->states[(int64_t)P.binder * N + b] = A.stnew[P.binder * cap + i];
-            } else {
-                int c0 = (mv == CHROMO_TANGENT_ROTATION) ? 3 : 0;
-                int c1 = (mv == CHROMO_SLIDE) ? 3 : 9;
-                for (int c = c0; c < c1; ++c) Pp[c * N + b] = A.bnew[c * cap + i];
-            }
-        }
-        if (check_field) {
-            double *rho = R->density;
-            for (int q = lane; q < U; q += WAVE) {
-                int64_t v = A.uvox[q];
-                for (int m = 0; m < Bp1; ++m)
-                    rho[v * Bp1 + m] = A.urho[m * A.ecap + q] + A.udrho[m * A.ecap + q];
-            }
-        }
-    }
-    // release the hash slots used by this move
-    if (check_field) {
-        for (int q = lane; q < U; q += WAVE) {
-            int s = A.uslot[q];
-            A.hkey[s] = HKEY_EMPTY;
-            A.hmin[s] = HMIN_EMPTY;
-        }
-    }
-    __syncthreads();
-    out.dEp = dEp;
-    out.dEf = dEf;
-    out.u = u;
-    out.expdE = expdE;
-    out.accepted = acc;
-    out.nbins = U;
-}
-
-// ---------------------------------------------------------------------------
-// mc_step: proposal (move_funcs.pyx), then move_body.  Wave-uniform until the
-// per-bead phase.
-// ---------------------------------------------------------------------------
-
-__device__ void load_pos3(const double *Pp, int64_t N, int64_t b, double v[3])
-{
-    v[0] = Pp[b];
-    v[1] = Pp[N + b];
-    v[2] = Pp[2 * N + b];
-}
-
-template <int NB>
-__device__ void mc_step_dev(const chromo_replica *R, WaveRng &rng, chromo_ctrl &C, MoveOut &out)
-{
-    const int lane = lane_id();
-    const int64_t N = R->N;
-    const double *Pp = R->pos;
-    Proposal P;
-    P.move = C.move;
-    P.binder = 0;
-    P.mypick = 0;
-    P.mynew = 0;
-    for (int q = 0; q < 9; ++q) P.myT[q] = 0;
-    int err = 0;
-    out.err = 0;
-    out.n = 0;
-    C.num_attempt += 1;   // MCAdapter.propose, moves.pyx:151
-    const double amp_move = C.amp_move;
-    const int64_t amp_bead = C.amp_bead;
-    int64_t n = 0, ind0 = 0, indf = 0;
-
-    if (P.move == CHROMO_CRANK_SHAFT) {
-        // crank_shaft, move_funcs.pyx:39-101
-        double ang = amp_move * (((double)rnd31(rng) / RMAX) - 0.5);
-        int64_t b0 = (int64_t)(int32_t)(((double)rnd31(rng) / RMAX) * (double)N);
-        int64_t b1 = from_point(rng, amp_bead, N, b0, err);
-        if (err) goto fail;
-        b1 = imax64(b1, 1);
-        check_bead_bounds(b0, b1, N, ind0, indf);
-        n = indf - ind0;
-        // get_crank_shaft_axis, move_funcs.pyx:157-234
-        int64_t ia, ib;
-        const int64_t indf_m1 = indf - 1, ind0_m1 = ind0 - 1;
-        if (ind0 == indf_m1 && ind0 == 0) { ia = indf; ib = ind0; }
-        else if (ind0 == indf_m1 && ind0 == N - 1) { ia = ind0; ib = ind0_m1; }
-        else if (ind0 == 0 && indf == N) { ia = indf_m1; ib = ind0; }
-        else if (ind0 == 0) { ia = indf; ib = ind0; }
-        else if (indf == N) { ia = indf_m1; ib = ind0_m1; }
-        else { ia = indf; ib = ind0_m1; }
-        double ra[3], rb[3], dir[3];
-        load_pos3(Pp, N, ia, ra);
-        load_pos3(Pp, N, ib, rb);
-        for (int i = 0; i < 3; ++i) dir[i] = ra[i] - rb[i];
-        double mag = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-        if (mag < 1E-5) {
-            sample_unit_sphere(rng, dir);
-        } else {
-            double sc = 1.0 / mag;
-            for (int i = 0; i < 3; ++i) dir[i] = dir[i] * sc;
-        }
-        // get_crank_shaft_fulcrum, move_funcs.pyx:237-280
-        int64_t ifc;
-        if (ind0 == 0 && indf != N) ifc = indf;
-        else if (ind0 != 0 && indf == N) ifc = ind0_m1;
-        else if (ind0 == 0 && indf == N) ifc = ind0;
-        else ifc = ind0_m1;
-        double pt[3];
-        load_pos3(Pp, N, ifc, pt);
-        axis_rotation(dir, pt, ang, P.T);
-    } else if (P.move == CHROMO_END_PIVOT) {
-        // end_pivot, move_funcs.pyx:285-346
-        double ang = amp_move * (((double)rnd31(rng) / RMAX) - 0.5);
-        int64_t lhs = rnd31(rng) % 2;
-        if (lhs == 1) {
-            ind0 = 0;
-            indf = from_left(rng, amp_bead, N, err) + 1;
-        } else {
-            ind0 = from_right(rng, amp_bead, N, err);
-            indf = N;
-        }
-        // a window of the whole chain would select bead N (an IndexError in the reference)
-        if (!err && indf > N) err = ERR_WINDOW;
-        if (err) goto fail;
-        n = indf - ind0;
-        double axis[3];
-        sample_unit_sphere(rng, axis);
-        // get_end_pivot_fulcrum, move_funcs.pyx:349-398
-        int64_t ifc;
-        if (ind0 == 0 && indf != N) ifc = indf;
-        else if (ind0 != 0 && indf == N) ifc = ind0 - 1;
-        else if (ind0 == 0 && indf == N && lhs == 1) ifc = indf - 1;
-        else ifc = ind0;
-        double pt[3];
-        load_pos3(Pp, N, ifc, pt);
-        axis_rotation(axis, pt, ang, P.T);
-    } else if (P.move == CHROMO_SLIDE) {
-        // slide, move_funcs.pyx:403-465
-        double amp = amp_move * ((double)rnd31(rng) / RMAX);
-        double dir[3];
-        sample_unit_sphere(rng, dir);
-        for (int i = 0; i < 3; ++i) dir[i] *= amp;
-        P.T[3] = dir[0];
-        P.T[7] = dir[1];
-        P.T[11] = dir[2];
-        int64_t b0 = rnd31(rng) % N;
-        int64_t b1 = from_point(rng, amp_bead, N, b0, err);
-        if (err) goto fail;
-        check_bead_bounds(b0, b1, N, ind0, indf);
-        n = indf - ind0;
-    } else if (P.move == CHROMO_TANGENT_ROTATION) {
-        // tangent_rotation + get_inds + rotate_select_beads, move_funcs.pyx:470-582
-        double ang = amp_move * (((double)rnd31(rng) / RMAX) - 0.5);
-        int64_t k = rnd31(rng) % amp_bead + 1;
-        if (k > N) {
-            err = ERR_TANGENT_K;
-            goto fail;
-        }
-        if (k > SEG_LDS && k > R->bead_cap) {
-            err = ERR_SEGCAP;
-            goto fail;
-        }
-        n = k;
-        Arena A;
-        if (k <= SEG_LDS) A = make_arena(lds_dyn, lds_layout(NB + 1));
-        else A = make_arena((char *)R->scratch, global_layout(R->bead_cap, R->seg_cap, NB + 1));
-        for (int64_t i = 0; i < k; ++i) {
-            for (;;) {
-                int64_t c = rnd31(rng) % N;
-                int dup = (lane < i && P.mypick == c);
-                for (int64_t j = WAVE + lane; j < i; j += WAVE) dup |= (A.inds[j] == c);
-                if (!__any(dup)) {
-                    if (i < WAVE) {
-                        if (lane == i) P.mypick = c;
-                    } else {
-                        if (lane == 0) A.inds[i] = c;
-                        __syncthreads();
-                    }
-                    break;
-                }
-            }
-        }
-        const double origin[3] = {0., 0., 0.};
-        for (int64_t i = 0; i < k; ++i) {
-            double dir[3], T[12];
-            sample_unit_sphere(rng, dir);
-            axis_rotation(dir, origin, ang, T);
-            if (i < WAVE) {
-                if (lane == i) {
-                    for (int j = 0; j < 3; ++j)
-                        for (int q = 0; q < 3; ++q) P.myT[3 * j + q] = T[4 * j + q];
-                }
-            } else if (lane == 0) {
-                for (int j = 0; j < 3; ++j)
-                    for (int q = 0; q < 3; ++q) A.tanT[(3 * j + q) * A.cap + i] = T[4 * j + q];
-            }
-        }
-        __syncthreads();
-        ind0 = 0;
-        indf = 0;
-    } else {
-        // change_binding_state, move_funcs.pyx:717-820
-        if (R->B < 1) {
-            err = ERR_NBINDERS;
-            goto fail;
-        }
-        int64_t binder = rnd31(rng) % NB;
-        int64_t ntails = R->sites[binder];
-        int64_t b0 = rnd31(rng) % N;
-        int64_t b1 = from_point(rng, amp_bead, N, b0, err);
-        if (err) goto fail;
-        check_bead_bounds(b0, b1, N, ind0, indf);
-        n = indf - ind0;
-        P.binder = (int)binder;
-        if (n > SEG_LDS && (n > R->seg_cap || n > R->bead_cap)) {
-            err = ERR_SEGCAP;
-            goto fail;
-        }
-        Arena A;
-        if (n <= SEG_LDS) A = make_arena(lds_dyn, lds_layout(NB + 1));
-        else A = make_arena((char *)R->scratch, global_layout(R->bead_cap, R->seg_cap, NB + 1));
-        for (int64_t i = 0; i < n; ++i) {
-            int st = (int)rnd_randint(rng, ntails + 1);
-            if (i < WAVE) {
-                if (lane == i) P.mynew = st;
-            } else if (lane == 0) {
-                A.stnew[binder * A.cap + i] = (int8_t)st;
-            }
-        }
-        __syncthreads();
-    }
-    P.n = n;
-    P.ind0 = ind0;
-    P.indf = indf;
-    out.n = n;
-    out.ind0 = ind0;
-    out.indf = indf;
-    if (n == 0) return;   // mc_sim.pyx:149-151: no dE, no Metropolis draw
-    if (P.move == CHROMO_TANGENT_ROTATION) {
-        // report the first picked bead
-        out.ind0 = __shfl(P.mypick, 0, WAVE);
-        out.indf = out.ind0 + 1;
-    }
-    if (n > SEG_LDS) {
-        const bool uses_field = R->field_on && P.move != CHROMO_TANGENT_ROTATION;
-        if (n > R->bead_cap || (uses_field && n > R->seg_cap)) {
-            err = ERR_SEGCAP;
-            goto fail;
-        }
-    }
-    if (n <= SEG_LDS) move_body<0, NB>(R, rng, P, out);
-    else move_body<1, NB>(R, rng, P, out);
-    // AcceptanceTracker.update_acceptance_rate (mc_stat.py:190-208)
-    if (out.accepted) {
-        C.num_success += 1;
-        C.rate = (C.alpha * 1.0) + (1 - C.alpha) * C.rate;
-    } else {
-        C.rate = (C.alpha * 0.0) + (1 - C.alpha) * C.rate;
-    }
-    return;
-fail:
-    out.err = err;
-    out.n = 0;
-}
-
-// SimpleControl.update_amplitudes (mc_controller.py:126-213)
-__device__ void update_amplitudes_dev(chromo_ctrl &c)
-{
-    if (!c.controlled) return;
-    double acc = c.rate;
-    if (acc < 0.5) {
-        double prop = c.amp_move * 0.95;
-        if (prop > c.move_lo) {
-            c.amp_move = prop;
-        } else {
-            c.amp_move = c.move_hi;
-            double nb = (double)(c.amp_bead - 1);
-            c.amp_bead = (int64_t)(c.bead_lo >= nb ? c.bead_lo : nb);
-        }
-    } else if (acc > 0.5) {
-        double prop = c.amp_move / 0.95;
-        if (prop < c.move_hi) {
-            c.amp_move = prop;
-        } else {
-            c.amp_move = c.move_lo;
-            double nb = (double)(c.amp_bead + 1);
-            c.amp_bead = (int64_t)(c.bead_hi <= nb ? c.bead_hi : nb);
-        }
-    }
-}
-
-// Algorithmic HBM bytes of one evaluated proposal.
-__device__ __forceinline__ int64_t move_bytes(int mv, int64_t n, int64_t U, int B, int acc)
-{
-    const int64_t Bp1 = B + 1;
-    if (mv == CHROMO_TANGENT_ROTATION) return 48 * n + 48 * 2 * n + (acc ? 48 * n : 0);
-    if (mv == CHROMO_CHANGE_BINDING_STATE)
-        return 2 * B + 24 + 8 * Bp1 * 8 + (acc ? (B + 8 * Bp1 * 8) : 0);
-    return 72 * n + 48 * 2 + B * n + 8 * Bp1 * U + (acc ? (72 * n + 8 * Bp1 * U) : 0);
-}
-
-// ---------------------------------------------------------------------------
-// Replica-level state load / store
-// ---------------------------------------------------------------------------
-
-__device__ void rng_load(const chromo_replica *R, WaveRng &rng, uint32_t *mt_lds, bool need_mt)
-{
-    const int lane = lane_id();
-    const uint32_t *w = R->rng;
-    rng.mode = R->rng_mode;
-    rng.g = lane < 31 ? w[CHROMO_RNG_GLIBC_STATE + lane] : 0u;
-    rng.gf = (int)w[CHROMO_RNG_GLIBC_F];
-    rng.gr = (int)w[CHROMO_RNG_GLIBC_R];
-    rng.mt = mt_lds;
-    rng.mti = (int)w[CHROMO_RNG_MT_INDEX];
-    if (need_mt && rng.mode == CHROMO_RNG_GLIBC) {
-        for (int i = lane; i < 624; i += WAVE) mt_lds[i] = w[CHROMO_RNG_MT_STATE + i];
-    }
-    rng.k0 = w[CHROMO_RNG_PHILOX_KEY];
-    rng.k1 = w[CHROMO_RNG_PHILOX_KEY + 1];
-    rng.ctr = (uint64_t)w[CHROMO_RNG_PHILOX_CTR] | ((uint64_t)w[CHROMO_RNG_PHILOX_CTR + 1] << 32);
-    rng.b0 = w[CHROMO_RNG_PHILOX_BUF];
-    rng.b1 = w[CHROMO_RNG_PHILOX_BUF + 1];
-    rng.b2 = w[CHROMO_RNG_PHILOX_BUF + 2];
-    rng.b3 = w[CHROMO_RNG_PHILOX_BUF + 3];
-    rng.bi = (int)w[CHROMO_RNG_PHILOX_IDX];
-    __syncthreads();
-}
-
-__device__ void rng_store(const chromo_replica *R, const WaveRng &rng, bool need_mt)
-{
-    const int lane = lane_id();
-    uint32_t *w = R->rng;
-    if (lane < 31) w[CHROMO_RNG_GLIBC_STATE + lane] = rng.g;
-    if (need_mt && rng.mode == CHROMO_RNG_GLIBC) {
-        __syncthreads();
-        for (int i = lane; i < 624; i += WAVE) w[CHROMO_RNG_MT_STATE + i] = rng.mt[i];
-    }
-    if (lane == 0) {
-        w[CHROMO_RNG_GLIBC_F] = (uint32_t)rng.gf;
-        w[CHROMO_RNG_GLIBC_R] = (uint32_t)rng.gr;
-        w[CHROMO_RNG_MT_INDEX] = (uint32_t)rng.mti;
-        w[CHROMO_RNG_PHILOX_CTR] = (uint32_t)rng.ctr;
-        w[CHROMO_RNG_PHILOX_CTR + 1] = (uint32_t)(rng.ctr >> 32);
-        w[CHROMO_RNG_PHILOX_BUF] = rng.b0;
-        w[CHROMO_RNG_PHILOX_BUF + 1] = rng.b1;
-        w[CHROMO_RNG_PHILOX_BUF + 2] = rng.b2;
-        w[CHROMO_RNG_PHILOX_BUF + 3] = rng.b3;
-        w[CHROMO_RNG_PHILOX_IDX] = (uint32_t)rng.bi;
-    }
-}
-
-__device__ void lds_hash_clear(int Bp1)
-{
-    ArenaLayout L = lds_layout(Bp1);
-    int32_t *hk = (int32_t *)(lds_dyn + L.hkey);
-    int32_t *hm = (int32_t *)(lds_dyn + L.hmin);
-    for (int i = lane_id(); i < HASH_LDS; i += WAVE) {
-        hk[i] = HKEY_EMPTY;
-        hm[i] = HMIN_EMPTY;
-    }
-    __syncthreads();
-}
-
-struct Counters {
-    int64_t prop, acc, sumn, sumb, ties, bytes, logged;
-};
-
-__device__ void record(const chromo_replica *R, const chromo_ctrl &C, const MoveOut &o, Counters &K)
-{
-    K.prop += 1;
-    K.acc += o.accepted;
-    K.sumn += o.n;
-    K.sumb += o.nbins;
-    K.ties += fabs(o.u - o.expdE) < 1E-12;
-    K.bytes += move_bytes(C.move, o.n, o.nbins, R->B, o.accepted);
-    if (R->log != nullptr && K.logged < R->log_cap) {
-        if (lane_id() == 0) {
-            chromo_move_log &L = R->log[K.logged];
-            L.move = C.move;
-            L.accepted = o.accepted;
-            L.n = o.n;
-            L.ind0 = o.ind0;
-            L.indf = o.indf;
-            L.dE_poly = o.dEp;
-            L.dE_field = o.dEf;
-            L.u = o.u;
-            L.exp_dE = o.expdE;
-            L.n_bins = o.nbins;
-        }
-        K.logged += 1;
-    }
-}
-
-__device__ void counters_load(const chromo_replica *R, Counters &K)
-{
-    const int64_t *c = R->counters;
-    K.prop = c[CHROMO_CNT_PROPOSALS];
-    K.acc = c[CHROMO_CNT_ACCEPTED];
-    K.sumn = c[CHROMO_CNT_SUM_N];
-    K.sumb = c[CHROMO_CNT_SUM_BINS];
-    K.ties = c[CHROMO_CNT_NEAR_TIES];
-    K.bytes = c[CHROMO_CNT_BYTES];
-    K.logged = c[CHROMO_CNT_LOGGED];
-}
-
-__device__ void counters_store(const chromo_replica *R, const Counters &K, int err)
-{
-    if (lane_id() != 0) return;
-    int64_t *c = R->counters;
-    c[CHROMO_CNT_PROPOSALS] = K.prop;
-    c[CHROMO_CNT_ACCEPTED] = K.acc;
-    c[CHROMO_CNT_SUM_N] = K.sumn;
-    c[CHROMO_CNT_SUM_BINS] = K.sumb;
-    c[CHROMO_CNT_NEAR_TIES] = K.ties;
-    c[CHROMO_CNT_BYTES] = K.bytes;
-    c[CHROMO_CNT_LOGGED] = K.logged;
-    if (err && c[CHROMO_CNT_ERROR] == 0) c[CHROMO_CNT_ERROR] = err;
-}
-
-__device__ void counters_error(const chromo_replica *R, int err)
-{
-    if (lane_id() == 0 && R->counters[CHROMO_CNT_ERROR] == 0) R->counters[CHROMO_CNT_ERROR] = err;
-}
-
-__device__ bool any_binding(const chromo_replica *R)
-{
-    for (int i = 0; i < R->n_ctrl; ++i)
-        if (R->ctrl[i].move == CHROMO_CHANGE_BINDING_STATE && R->ctrl[i].on == 1) return true;
-    return false;
-}
-
-// ---------------------------------------------------------------------------
-// Kernels
-// ---------------------------------------------------------------------------
-
-template <int NB>
-__device__ void run_steps(const chromo_replica *R, WaveRng &rng, Counters &K, chromo_ctrl *sctrl,
-                          int nc, int64_t steps, int &err)
-{
-    for (int64_t s = 0; s < steps && !err; ++s) {
-        for (int ci = 0; ci < nc && !err; ++ci) {
-            chromo_ctrl c = sctrl[ci];
-            if (c.on == 1) {
-                for (int j = 0; j < c.num_per_cycle; ++j) {
-                    MoveOut o;
-                    mc_step_dev<NB>(R, rng, c, o);
-                    if (o.err) {
-                        err = o.err;
-                        break;
-                    }
-                    if (o.n > 0) record(R, c, o, K);
-                }
-            }
-            if (!err) update_amplitudes_dev(c);
-            __syncthreads();
-            if (lane_id() == 0) sctrl[ci] = c;
-            __syncthreads();
-        }
-    }
-}
 
 // Persistent mc_sim: one workgroup (one wave) per replica.
 __global__ void __launch_bounds__(64) k_mc_run(const chromo_replica *reps, int n_reps, int64_t steps,
@@ -1035,6 +96,17 @@ __global__ void __launch_bounds__(64) k_mc_step(const chromo_replica *reps, int 
     rng_store(R, rng, need_mt);
     if (lane_id() == 0) R->ctrl[ci] = C;
     counters_store(R, K, o.err);
+}
+
+// Flags replicas whose binder count is not in mask (bit B set for B present).
+__global__ void k_check_binders(const chromo_replica *reps, int n_reps, int mask)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_reps) return;
+    const chromo_replica *R = &reps[r];
+    const int B = R->B;
+    if ((B < 1 || B > CHROMO_MAX_BINDERS || !(mask & (1 << B))) && R->counters[CHROMO_CNT_ERROR] == 0)
+        R->counters[CHROMO_CNT_ERROR] = ERR_NBINDERS;
 }
 
 __global__ void k_update_amplitudes(const chromo_replica *reps, int rep, int ci)
@@ -1149,7 +221,7 @@ GridGeom grid_of(const chromo_replica &r)
 // ---- tape: field dE of a host-supplied trial configuration ----------------
 
 template <int NB>
-__device__ void field_tape_body(const chromo_replica *R, const int64_t *inds, int64_t n,
+__device__ __forceinline__ void field_tape_body(const chromo_replica *R, const int64_t *inds, int64_t n,
                                 const double *r_trial, const int8_t *st_trial, int state_change,
                                 double *dE_out, int64_t *bins, double *delta, int64_t *nbins)
 {
@@ -1515,6 +587,44 @@ int chromo_mc_run(const chromo_replica *d_reps, int32_t n_reps, int64_t num_step
     hipLaunchKernelGGL(k_mc_run, dim3(n_reps), dim3(64), g_lds_bytes, (hipStream_t)stream, d_reps,
                        n_reps, num_steps, g_lds_bp1);
     HIPC(launch_err());
+    return 0;
+}
+
+int chromo_mc_run_tuned(const chromo_replica *d_reps, int32_t n_reps, int64_t num_steps,
+                        int32_t waves_per_replica, int32_t binder_mask, void *stream)
+{
+    int rc = check_reps(d_reps, n_reps);
+    if (rc) return rc;
+    if (num_steps < 0) return set_err(-1, "num_steps < 0");
+    if ((binder_mask & 0x1e) == 0 || (binder_mask & ~0x1e) != 0)
+        return set_err(-1, "binder_mask must set only bits 1..4 (binder counts present)");
+    const int W = waves_per_replica;
+    if (W == 8)
+        return set_err(-1, "waves_per_replica = 8 is disabled: its build faulted the GPU with an "
+                           "out-of-aperture access whose cause is not found yet");
+    if (W != 1 && W != 2 && W != 4) return set_err(-1, "waves_per_replica must be 1, 2 or 4");
+    rc = ensure_lds();
+    if (rc) return rc;
+    int bp1 = 0;
+    for (int b = 1; b <= CHROMO_MAX_BINDERS; ++b)
+        if (binder_mask & (1 << b)) bp1 = b + 1;
+    if (bp1 > g_lds_bp1) return set_err(-1, "binder count exceeds what the device LDS holds");
+    hipStream_t st = (hipStream_t)stream;
+    // replicas whose binder count is not in binder_mask fail with ERR_NBINDERS
+    hipLaunchKernelGGL(k_check_binders, dim3((n_reps + 63) / 64), dim3(64), 0, st, d_reps, n_reps,
+                       binder_mask);
+    HIPC(launch_err());
+    if (W == 1) {
+        hipLaunchKernelGGL(k_mc_run, dim3(n_reps), dim3(WAVE), lds_bytes_for(bp1), st, d_reps, n_reps,
+                           num_steps, bp1);
+        HIPC(launch_err());
+        return 0;
+    }
+    hipError_t e = W == 2 ? chromo::spec_launch_w2(d_reps, n_reps, num_steps, binder_mask, st)
+                          : chromo::spec_launch_w4(d_reps, n_reps, num_steps, binder_mask, st);
+    if (e == hipErrorInvalidValue)
+        return set_err(-1, "LDS of this launch shape exceeds the device limit");
+    HIPC(e);
     return 0;
 }
 

@@ -149,6 +149,8 @@ class DeviceReplicas:
     """Chains, fields, controllers and RNG streams of ``n`` replicas in HBM."""
 
     CTRL_SLOTS = 8
+    WAVE_CHOICES = (1, 2, 4)
+    waves = 1          # wavefronts per replica used by run(); see chromo_amd.autotune
 
     def __init__(self, chains: Sequence[ChainSpec], fields: Sequence[FieldSpec],
                  ctrls: Sequence[Sequence[CtrlSpec]], *, rng_mode: int = L.RNG_GLIBC,
@@ -181,6 +183,7 @@ class DeviceReplicas:
             if f.field_on and f.n_bins >= 2 ** 31:
                 raise NotImplementedError("more than 2^31 voxels")
         self.Ns, self.Bs = Ns, Bs
+        self.binder_mask = sum(1 << b for b in set(Bs))
         nctrl = [len(c) for c in ctrls]
         if max(nctrl) > self.CTRL_SLOTS:
             raise NotImplementedError(f"at most {self.CTRL_SLOTS} controllers per replica")
@@ -340,10 +343,13 @@ class DeviceReplicas:
             self._tab_host[i].mu_adjust = float(v)
         self._upload_table()
 
-    def run(self, num_steps: int, *, sync: bool = True) -> None:
-        """mc_sim for every replica in one persistent launch."""
-        L.check(self.lib.chromo_mc_run(self.tab_ptr, self.n, int(num_steps), _current_stream()),
-                "chromo_mc_run")
+    def run(self, num_steps: int, *, sync: bool = True, waves: Optional[int] = None) -> None:
+        """mc_sim for every replica in one persistent launch, with ``waves``
+        wavefronts per replica (default: ``self.waves``).  The result does not
+        depend on ``waves``."""
+        w = int(self.waves if waves is None else waves)
+        L.check(self.lib.chromo_mc_run_tuned(self.tab_ptr, self.n, int(num_steps), w, self.binder_mask,
+                                             _current_stream()), "chromo_mc_run_tuned")
         if sync:
             self.raise_on_error()
 

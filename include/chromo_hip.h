@@ -172,6 +172,17 @@ int chromo_density_init(const chromo_replica *d_reps, int32_t n_reps, void *stre
  * then update_amplitudes per controller.  Chains advance independently. */
 int chromo_mc_run(const chromo_replica *d_reps, int32_t n_reps, int64_t num_steps,
                   void *stream);
+/* chromo_mc_run with an explicit launch shape.  waves_per_replica = 1 runs each
+ * chain on one wavefront, one proposal at a time; 2, 4 or 8 wavefronts evaluate
+ * that many proposals of a controller block at once and resolve them in order,
+ * re-evaluating any proposal an earlier accepted one in the same round touched,
+ * so the chains (decisions, states, densities, RNG streams) are the same for
+ * every waves_per_replica.  binder_mask has bit B set for each binder count B
+ * (1..4) among the replicas; a replica whose B is not in it fails with the
+ * binder-count error.  No reference counterpart: it is a launch-shape variant
+ * of mc_sim (chromo/mc/mc_sim.pyx:26-103). */
+int chromo_mc_run_tuned(const chromo_replica *d_reps, int32_t n_reps, int64_t num_steps,
+                        int32_t waves_per_replica, int32_t binder_mask, void *stream);
 /* The same schedule for one replica with one kernel launch per mc_step and one
  * per controller update (the "plain" path). */
 int chromo_mc_run_plain(const chromo_replica *d_reps, int32_t rep, int64_t num_steps,

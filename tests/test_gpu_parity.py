@@ -17,6 +17,10 @@ from chromo_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
+# Eight wavefronts per replica: a benchmark-size run of that build faulted the GPU
+# (out-of-aperture access) and the cause is not found; the entry point rejects it.
+W8_SKIP = "waves_per_replica = 8 disabled after a GPU fault with no cause found yet"
+
 
 def say(*a):
     print(*a, flush=True)
@@ -107,15 +111,17 @@ def test_field_tape(oracle, n_move, seed):
 
 # ---------------------------------------------------------------- mc_sim ---
 
-def _compare_run(oracle, s, steps, inner_seed=123, rng_mode=0, seed=1, stream=0, launch="run"):
+def _compare_run(oracle, s, steps, inner_seed=123, rng_mode=0, seed=1, stream=0, launch="run",
+                 waves=1):
     ctrls_o = copy.deepcopy(s.ctrls)
     o = oracle.OracleSim(s.chain, s.field, rng_mode=rng_mode, seed=seed, stream=stream)
     o.np_seed(inner_seed)
     lo = o.mc_sim(steps, ctrls_o, max_logs=10 ** 6)
     dr = _dev([s], rng_mode=rng_mode, seeds=[seed], streams=[stream], log_cap=len(lo) + 100)
     dr.np_seed([inner_seed])
+    say("  waves", waves, "launch", launch, "proposals", len(lo))
     if launch == "run":
-        dr.run(steps)
+        dr.run(steps, waves=waves)
     else:
         dr.run_plain(steps)
     lg = dr.logs(0)
@@ -129,8 +135,10 @@ def _compare_run(oracle, s, steps, inner_seed=123, rng_mode=0, seed=1, stream=0,
     r, t3, t2 = dr.positions(0)
     ro, t3o, t2o, sto = o.state()
     assert np.array_equal(dr.binder_states(0), sto)
-    assert np.max(np.abs(r - ro)) < 1e-9 and np.max(np.abs(t3 - t3o)) < 1e-12
-    assert np.max(np.abs(t2 - t2o)) < 1e-12
+    # unit tangents: the same 1e-9 bound (ulp differences of the device's
+    # sin/cos/acos accumulate over repeated rotations of one bead)
+    assert np.max(np.abs(r - ro)) < 1e-9 and np.max(np.abs(t3 - t3o)) < 1e-9
+    assert np.max(np.abs(t2 - t2o)) < 1e-9
     assert np.array_equal(_lower_voxels(s.field, r), _lower_voxels(s.field, ro))
     # densities: per-voxel folds of weights whose positions may differ by ulps;
     # tolerance 1e-9 of one bead's density per voxel (1 / vol_bin)
@@ -146,37 +154,110 @@ def _compare_run(oracle, s, steps, inner_seed=123, rng_mode=0, seed=1, stream=0,
     return lo
 
 
+@pytest.mark.parametrize("waves", [1, 4])
 @pytest.mark.parametrize("N,nvox,seed", [(300, 8, 11), (1000, 30, 12), (5000, 20, 13)])
-def test_mc_sim_glibc_parity(oracle, N, nvox, seed):
+def test_mc_sim_glibc_parity(oracle, N, nvox, seed, waves):
     say("mc_sim glibc", N, nvox, seed)
     s = W.make_system(N, 2, nvox, seed=seed)
-    lo = _compare_run(oracle, s, steps=15)
+    lo = _compare_run(oracle, s, steps=15, waves=waves)
     assert lo["accepted"].sum() > 100
 
 
-def test_mc_sim_philox_parity(oracle):
+@pytest.mark.parametrize("waves", [1, 2, pytest.param(8, marks=pytest.mark.skip(reason=W8_SKIP))])
+def test_mc_sim_philox_parity(oracle, waves):
     say("mc_sim philox")
     s = W.make_system(800, 2, 12, seed=21, chi=0.7, eps=[-4.5, -1.5], mu=[-1.0, 0.5])
-    _compare_run(oracle, s, steps=12, rng_mode=1, seed=777, stream=5)
+    _compare_run(oracle, s, steps=12, rng_mode=1, seed=777, stream=5, waves=waves)
 
 
-def test_mc_sim_three_binders_and_cross_talk(oracle):
+@pytest.mark.parametrize("waves", [1, 4])
+def test_mc_sim_three_binders_and_cross_talk(oracle, waves):
     say("mc_sim B=3")
     s = W.make_system(600, 3, 10, seed=22, eps=[-4, -3, -2], cross_eps=0.5, mu=[-0.4, 0.2, -1])
-    _compare_run(oracle, s, steps=10)
+    _compare_run(oracle, s, steps=10, waves=waves)
 
 
-def test_mc_sim_large_moves_use_global_arena(oracle):
+@pytest.mark.parametrize("B,waves", [(1, 4), (4, 4), pytest.param(4, 8, marks=pytest.mark.skip(reason=W8_SKIP))])
+def test_mc_sim_one_and_four_binders(oracle, B, waves):
+    say("mc_sim B", B)
+    s = W.make_system(700, B, 10, seed=40 + B, eps=-3.0, cross_eps=0.3 if B > 1 else 0.0,
+                      mu=-0.5)
+    _compare_run(oracle, s, steps=8, waves=waves)
+
+
+@pytest.mark.parametrize("waves", [1, 4])
+def test_mc_sim_large_moves_use_global_arena(oracle, waves):
     """Crank and slide windows up to 150 beads: moves above 32 beads run through
-    the global scratch arena."""
+    the global scratch arena (and, with several wavefronts, one at a time)."""
     say("mc_sim large moves")
     s = W.make_system(1500, 2, 16, seed=23)
     for c in s.ctrls:
         if c.move in (L.CRANK_SHAFT, L.SLIDE, L.END_PIVOT):
             c.amp_bead = int(c.bead_hi)
             c.bead_lo = c.bead_hi
-    lo = _compare_run(oracle, s, steps=6)
+    lo = _compare_run(oracle, s, steps=6, waves=waves)
     assert lo["n"].max() > 32
+
+
+@pytest.mark.parametrize("waves", [2, 4, pytest.param(8, marks=pytest.mark.skip(reason=W8_SKIP))])
+def test_mc_sim_short_chain_conflicts(oracle, waves):
+    """A 40-bead chain on a 3^3 grid: proposals of one round overlap in beads and
+    voxels all the time, so most rounds re-evaluate."""
+    say("mc_sim short chain")
+    s = W.make_system(40, 2, 3, seed=25)
+    lo = _compare_run(oracle, s, steps=25, waves=waves)
+    assert lo["accepted"].sum() > 50
+
+
+@pytest.mark.parametrize("nvox,waves", [(1, 4), (2, 4), (1, 1)])
+def test_mc_sim_tiny_grids(oracle, nvox, waves):
+    """Grids of one or two voxels per side: corners wrap onto the same voxel."""
+    say("mc_sim tiny grid", nvox)
+    s = W.make_system(120, 2, nvox, seed=26, chi=0.1)
+    s.field.vf_limit = 1e9
+    _compare_run(oracle, s, steps=6, waves=waves)
+
+
+@pytest.mark.parametrize("waves,rng_mode", [(1, 0), (4, 0), (2, 1), pytest.param(8, 1, marks=pytest.mark.skip(reason=W8_SKIP))])
+def test_mc_sim_coincident_beads_redraw_crank_axis(oracle, waves, rng_mode):
+    """Beads 200..260 sit on one point, so crank-shaft axes there are shorter
+    than 1e-5 and the reference draws a random axis (two extra draws); with
+    several wavefronts such a proposal is redone one at a time from its RNG state."""
+    say("mc_sim coincident beads")
+    s = W.make_system(600, 2, 12, seed=27)
+    s.chain.r[200:261] = s.chain.r[200]
+    # 61 beads in one voxel exceed vf_limit; keep the 1e99 penalty (whose
+    # cancellation residue depends on summation order) out of this test
+    s.field.vf_limit = 1e9
+    lo = _compare_run(oracle, s, steps=10, waves=waves, rng_mode=rng_mode, seed=3)
+    crank = lo[lo["move"] == L.CRANK_SHAFT]
+    assert np.any((crank["ind0"] >= 200) & (crank["indf"] <= 261))
+
+
+def test_all_wave_counts_identical():
+    """Every enabled launch shape gives the same chains bit for bit: logs (dE included),
+    positions, states, densities, RNG streams and controllers."""
+    say("waves 1/2/4/8 identical")
+    systems = [W.make_system(n, 2, g, seed=50 + i, chi=0.4 + 0.3 * i)
+               for i, (n, g) in enumerate([(3000, 16), (800, 10), (12000, 24)])]
+    runs = []
+    for w in (1, 2, 4):   # 8: disabled, see W8_SKIP
+        dr = _dev(systems, rng_mode=1, seeds=[9] * 3, streams=[0, 1, 2], log_cap=20000)
+        dr.run(12, waves=w)
+        runs.append(dr)
+    base = runs[0]
+    for dr in runs[1:]:
+        for i in range(3):
+            la, lb = base.logs(i), dr.logs(i)
+            assert la.tobytes() == lb.tobytes()
+            for a, b in zip(base.positions(i), dr.positions(i)):
+                assert np.array_equal(a, b)
+            assert np.array_equal(base.binder_states(i), dr.binder_states(i))
+            assert np.array_equal(base.density_of(i), dr.density_of(i))
+            assert np.array_equal(base.rng_words(i), dr.rng_words(i))
+            for ca, cb in zip(base.controllers(i), dr.controllers(i)):
+                assert ca == cb
+        assert np.array_equal(base.counter_array(), dr.counter_array())
 
 
 def test_plain_equals_persistent(oracle):
