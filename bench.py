@@ -4,7 +4,10 @@ Headline workload (``value``): 512 independent replicas advanced together by
 the persistent kernel, chain lengths N ~ U{2000..50000}, 32^3..64^3 voxels of
 30 nm, per-replica chi ~ U[0,2], eps ~ U[-5,-1], mu ~ U[-2,1], Bernoulli(0.3)
 marks, Philox streams.  One *step* is one mc_sim sweep of every replica
-(all_moves: 161 proposals per replica) in one kernel launch.
+(all_moves: 161 proposals per replica) in one kernel launch.  The launch shape
+(wavefronts per replica) is chosen by chromo_amd.autotune after the warmup; all
+shapes produce the same chains.  The timed sweeps replay a captured HIP graph
+of one sweep (``--no-graph``: direct launches).
 
 Also reported: the plain path (one kernel launch per proposal) on the
 100,000-bead / 64^3 configuration, the ratio of the two, a dE / decision check
@@ -65,10 +68,23 @@ def bench_batched(args):
     log(f"batched: {len(systems)} replicas built in {time.time() - t0:.1f}s "
         f"(beads {sum(s.chain.N for s in systems)})")
     for _ in range(args.warmup):
-        dr.run(1, sync=False)
+        dr.run(1, sync=False, waves=1)
     dr.raise_on_error()
+    if args.waves == "auto":
+        from chromo_amd.autotune import autotune
+        tune = autotune(dr, sweeps=2)
+        log("autotune (s/sweep): " + ", ".join(f"W={w}: {t * 1e3:.3f} ms" for w, t in tune.items())
+            + f" -> W={dr.waves}")
+    else:
+        tune = None
+        dr.waves = int(args.waves)
     c0 = dr.counter_array()
-    secs = timed(lambda: dr.run(1, sync=False), args.steps)
+    if args.no_graph:
+        secs = timed(lambda: dr.run(1, sync=False), args.steps)
+    else:
+        g = dr.capture(1)               # one sweep of every replica per graph launch
+        c0 = dr.counter_array()
+        secs = timed(g.replay, args.steps)
     dr.raise_on_error()
     c1 = dr.counter_array()
     d = c1 - c0
@@ -76,7 +92,8 @@ def bench_batched(args):
     nbytes = int(d[:, L.CNT_BYTES].sum())
     return dict(secs=secs, proposals=props, bytes=nbytes, accepted=int(d[:, L.CNT_ACCEPTED].sum()),
                 near_ties=int(d[:, L.CNT_NEAR_TIES].sum()), replicas=len(systems),
-                beads=int(sum(s.chain.N for s in systems)), systems=systems)
+                beads=int(sum(s.chain.N for s in systems)), systems=systems, waves=dr.waves,
+                tune=tune)
 
 
 def bench_plain(args):
@@ -94,8 +111,8 @@ def bench_plain(args):
                 bytes=int(d[L.CNT_BYTES]))
 
 
-def dE_check(systems, sweeps=20, n=4):
-    """GPU vs oracle on the first n replicas of the batched workload."""
+def dE_check(systems, sweeps=20, n=4, waves=1):
+    """GPU (launch shape ``waves``) vs oracle on the first n replicas of the batched workload."""
     from chromo_amd import _lib as L
     from oracle import oracle as O
     worst, mism, total = 0.0, 0, 0
@@ -104,7 +121,7 @@ def dE_check(systems, sweeps=20, n=4):
         o = O.OracleSim(s.chain, s.field, rng_mode=O.RNG_PHILOX, seed=20240601, stream=i)
         lo = o.mc_sim(sweeps, copy.deepcopy(s.ctrls), max_logs=10 ** 6)
         dr = build_replicas([s], L.RNG_PHILOX, [20240601], [i], log_cap=len(lo) + 16)
-        dr.run(sweeps)
+        dr.run(sweeps, waves=waves)
         lg = dr.logs(0)
         if len(lg) != len(lo):
             mism += abs(len(lg) - len(lo))
@@ -146,6 +163,10 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch each sweep directly instead of replaying a captured HIP graph")
+    ap.add_argument("--waves", default="auto",
+                    help="wavefronts per replica (1, 2, 4) or 'auto' to time each and keep the fastest")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: replicas are independent chains; multi-GPU runs "
@@ -161,7 +182,7 @@ def main():
     p = bench_plain(args)
     plain_value = p["proposals"] / p["secs"]
     log(f"plain: {p['proposals']} proposals in {p['secs']:.3f}s -> {plain_value:.4g}/s")
-    chk = None if args.no_check else dE_check(b["systems"])
+    chk = None if args.no_check else dE_check(b["systems"], waves=b["waves"])
     if chk:
         log(f"check: {chk}")
     cpu = None if args.no_cpu_baseline else cpu_baseline(b["systems"])
@@ -194,7 +215,11 @@ def main():
                         f"({b['beads']} beads), 32^3-64^3 voxels of 30 nm, B=2, all_moves + "
                         "SimpleControl, Philox; one step = one sweep (161 proposals per replica)",
             "replicas": b["replicas"],
-            "parallelism": "replicas (one wavefront per replica)",
+            "waves_per_replica": b["waves"],
+            "parallelism": (f"replicas, one workgroup each; {b['waves']} wavefront(s) per replica "
+                            "evaluating proposals of a controller block at once"),
+            "autotune_ms_per_sweep": ({str(w): t * 1e3 for w, t in b["tune"].items()}
+                                      if b["tune"] else None),
         },
         "plain": {"value": plain_value, "unit": "move-evals/s",
                   "workload": "100,000 beads, 64^3 voxels, B=2, one kernel launch per proposal",
@@ -205,7 +230,7 @@ def main():
         "dE_check": chk,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_mc_run",
+                     "kernel": "k_mc_run" if b["waves"] == 1 else f"k_mc_spec<{b['waves']}, 2>",
                      "algorithmic_bytes_per_launch": b["bytes"] / args.steps},
         "cpu_baseline": cpu,
     }

@@ -28,7 +28,7 @@ ROOT = HERE.parent
 CSRC = HERE / "csrc"
 OUT = HERE / "libchromo_hip.so"
 ARCH = os.environ.get("CHROMO_OFFLOAD_ARCH", "gfx950")
-SPEC_WAVES = (2, 4)
+SPEC_WAVES = (2,)
 
 # (source, extra flags, object name)
 UNITS = [(CSRC / "chromo_hip.hip", [], "chromo_hip.o"),

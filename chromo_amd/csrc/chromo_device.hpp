@@ -63,44 +63,125 @@ __device__ __forceinline__ void wave_sync_global()
 // Wave-level helpers
 // ---------------------------------------------------------------------------
 
+// Lane l's value as a wave-uniform scalar (l must be wave-uniform).
 __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int l)
 {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
 }
 
-__device__ __forceinline__ double bcast0(double x) { return __shfl(x, 0, WAVE); }
-__device__ __forceinline__ int64_t bcast0(int64_t x) { return __shfl(x, 0, WAVE); }
-__device__ __forceinline__ int bcast0(int x) { return __shfl(x, 0, WAVE); }
+__device__ __forceinline__ int64_t readlane_i64(int64_t v, int l)
+{
+    uint32_t lo = readlane_u32((uint32_t)v, l);
+    uint32_t hi = readlane_u32((uint32_t)((uint64_t)v >> 32), l);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
 
-// Fixed-order butterfly sum; every lane returns lane 0's result.
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    return __longlong_as_double((long long)readlane_i64((int64_t)__double_as_longlong(v), l));
+}
+
+__device__ __forceinline__ double bcast0(double x) { return readlane_f64(x, 0); }
+__device__ __forceinline__ int64_t bcast0(int64_t x) { return readlane_i64(x, 0); }
+__device__ __forceinline__ int bcast0(int x) { return (int)readlane_u32((uint32_t)x, 0); }
+
+// DPP lane permutations within rows of 16 lanes (VALU operations, no LDS trip).
+constexpr int DPP_XOR1 = 0xB1;          // quad_perm [1,0,3,2]
+constexpr int DPP_XOR2 = 0x4E;          // quad_perm [2,3,0,1]
+constexpr int DPP_HALF_MIRROR = 0x141;  // lane i <-> 7 - i within each 8
+constexpr int DPP_MIRROR = 0x140;       // lane i <-> 15 - i within each 16
+constexpr int DPP_SHR1 = 0x111, DPP_SHR2 = 0x112, DPP_SHR4 = 0x114, DPP_SHR8 = 0x118;
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = dpp_i32<CTRL>((int)b);
+    const int hi = dpp_i32<CTRL>((int)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+template <int CTRL>
+__device__ __forceinline__ int64_t dpp_i64(int64_t x)
+{
+    const int lo = dpp_i32<CTRL>((int)x);
+    const int hi = dpp_i32<CTRL>((int)(x >> 32));
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// Sum over all 64 lanes in a fixed order: pairwise within rows of 16 (every lane
+// of a row ends with the same row sum), then the four row sums left to right.
+// Wave-uniform result.  Called with all lanes active.
 __device__ __forceinline__ double wave_sum(double x)
 {
-    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, WAVE);
-    return bcast0(x);
+    x += dpp_f64<DPP_XOR1>(x);
+    x += dpp_f64<DPP_XOR2>(x);
+    x += dpp_f64<DPP_HALF_MIRROR>(x);
+    x += dpp_f64<DPP_MIRROR>(x);
+    const double r0 = readlane_f64(x, 0), r1 = readlane_f64(x, 16), r2 = readlane_f64(x, 32),
+                 r3 = readlane_f64(x, 48);
+    return ((r0 + r1) + r2) + r3;
 }
 
 __device__ __forceinline__ int64_t wave_sum_i64(int64_t x)
 {
-    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, WAVE);
-    return x;
+    x += dpp_i64<DPP_XOR1>(x);
+    x += dpp_i64<DPP_XOR2>(x);
+    x += dpp_i64<DPP_HALF_MIRROR>(x);
+    x += dpp_i64<DPP_MIRROR>(x);
+    return readlane_i64(x, 0) + readlane_i64(x, 16) + readlane_i64(x, 32) + readlane_i64(x, 48);
 }
 
 __device__ __forceinline__ int wave_sum_int(int x)
 {
-    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, WAVE);
-    return x;
+    x += dpp_i32<DPP_XOR1>(x);
+    x += dpp_i32<DPP_XOR2>(x);
+    x += dpp_i32<DPP_HALF_MIRROR>(x);
+    x += dpp_i32<DPP_MIRROR>(x);
+    return (int)(readlane_u32((uint32_t)x, 0) + readlane_u32((uint32_t)x, 16) + readlane_u32((uint32_t)x, 32) +
+                 readlane_u32((uint32_t)x, 48));
+}
+
+// Sum over lanes 0..7 (wave-uniform), for values held by the first eight lanes.
+__device__ __forceinline__ int64_t sum_lanes8_i64(int64_t x)
+{
+    x += dpp_i64<DPP_XOR1>(x);
+    x += dpp_i64<DPP_XOR2>(x);
+    x += dpp_i64<DPP_HALF_MIRROR>(x);
+    return readlane_i64(x, 0);
+}
+
+__device__ __forceinline__ uint32_t wave_or(uint32_t m)
+{
+    int x = (int)m;
+    x |= dpp_i32<DPP_XOR1>(x);
+    x |= dpp_i32<DPP_XOR2>(x);
+    x |= dpp_i32<DPP_HALF_MIRROR>(x);
+    x |= dpp_i32<DPP_MIRROR>(x);
+    return readlane_u32((uint32_t)x, 0) | readlane_u32((uint32_t)x, 16) | readlane_u32((uint32_t)x, 32) |
+           readlane_u32((uint32_t)x, 48);
 }
 
 // Exclusive prefix sum over the wave; total receives the wave sum.
 __device__ __forceinline__ int wave_excl_scan(int x, int &total)
 {
+    // inclusive scan within rows of 16 (lanes shifted in from outside the row read 0)
     int v = x;
-    const int lane = lane_id();
-    for (int d = 1; d < WAVE; d <<= 1) {
-        int y = __shfl_up(v, d, WAVE);
-        if (lane >= d) v += y;
-    }
-    total = __shfl(v, WAVE - 1, WAVE);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_SHR1, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_SHR2, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_SHR4, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_SHR8, 0xF, 0xF, true);
+    const int s0 = (int)readlane_u32((uint32_t)v, 15), s1 = (int)readlane_u32((uint32_t)v, 31),
+              s2 = (int)readlane_u32((uint32_t)v, 47), s3 = (int)readlane_u32((uint32_t)v, 63);
+    const int row = lane_id() >> 4;
+    v += (row > 0 ? s0 : 0) + (row > 1 ? s1 : 0) + (row > 2 ? s2 : 0);
+    total = s0 + s1 + s2 + s3;
     return v - x;
 }
 

@@ -19,7 +19,6 @@ using namespace chromo;
 
 namespace chromo {
 hipError_t spec_launch_w2(const chromo_replica *, int, int64_t, int, hipStream_t);
-hipError_t spec_launch_w4(const chromo_replica *, int, int64_t, int, hipStream_t);
 }  // namespace chromo
 
 namespace {
@@ -248,8 +247,10 @@ __device__ __forceinline__ void field_tape_body(const chromo_replica *R, const i
             double wt[3];
             voxel_lower(R, pos, ix, wt);
             const int p = (int)(2 * i + k);
-            for (int j = 0; j < 3; ++j) A.pw[j * 2 * cap + p] = wt[j];
-            for (int l = 0; l < 8; ++l) A.evox[p * 8 + l] = (int32_t)corner_voxel(R, ix, l);
+            for (int l = 0; l < 8; ++l) {
+                A.evox[p * 8 + l] = (int32_t)corner_voxel(R, ix, l);
+                A.c0[p * 8 + l] = corner_weight(l, wt[0], wt[1], wt[2]) / R->vol_bin;
+            }
         }
     }
     __syncthreads();
@@ -325,10 +326,8 @@ __device__ __forceinline__ void field_tape_body(const chromo_replica *R, const i
         for (int m = 0; m < Bp1; ++m) acc[m] = 0;
         for (int a = s0; a < s1; ++a) {
             const int e = A.list[a];
-            const int p = e >> 3, l = e & 7;
-            const int i = p >> 1, k = p & 1;
-            const double wt = corner_weight(l, A.pw[p], A.pw[2 * cap + p], A.pw[4 * cap + p]);
-            const double c0 = wt / vol_bin;
+            const int i = e >> 4, k = (e >> 3) & 1;
+            const double c0 = A.c0[e];
             const double pre = k ? 1. : -1.;
             const double t0 = pre * c0;
             if (fabs(t0) > 1E-18) acc[0] += t0;
@@ -599,10 +598,11 @@ int chromo_mc_run_tuned(const chromo_replica *d_reps, int32_t n_reps, int64_t nu
     if ((binder_mask & 0x1e) == 0 || (binder_mask & ~0x1e) != 0)
         return set_err(-1, "binder_mask must set only bits 1..4 (binder counts present)");
     const int W = waves_per_replica;
-    if (W == 8)
-        return set_err(-1, "waves_per_replica = 8 is disabled: its build faulted the GPU with an "
-                           "out-of-aperture access whose cause is not found yet");
-    if (W != 1 && W != 2 && W != 4) return set_err(-1, "waves_per_replica must be 1, 2 or 4");
+    if (W == 4 || W == 8)
+        return set_err(-1, "waves_per_replica = 4 and 8 are disabled: those builds spill heavily "
+                           "and gave a GPU fault (8), a hang and wrong voxel counts (4) whose "
+                           "cause is not found yet");
+    if (W != 1 && W != 2) return set_err(-1, "waves_per_replica must be 1 or 2");
     rc = ensure_lds();
     if (rc) return rc;
     int bp1 = 0;
@@ -620,8 +620,7 @@ int chromo_mc_run_tuned(const chromo_replica *d_reps, int32_t n_reps, int64_t nu
         HIPC(launch_err());
         return 0;
     }
-    hipError_t e = W == 2 ? chromo::spec_launch_w2(d_reps, n_reps, num_steps, binder_mask, st)
-                          : chromo::spec_launch_w4(d_reps, n_reps, num_steps, binder_mask, st);
+    hipError_t e = chromo::spec_launch_w2(d_reps, n_reps, num_steps, binder_mask, st);
     if (e == hipErrorInvalidValue)
         return set_err(-1, "LDS of this launch shape exceeds the device limit");
     HIPC(e);

@@ -26,9 +26,9 @@ using namespace chromo;
 // ---------------------------------------------------------------------------
 
 struct ArenaLayout {
-    size_t bnew, tanT, epart, pw, udrho, urho;     // double arrays
+    size_t bnew, tanT, epart, c0, udrho, urho;     // double arrays
     size_t inds;                                   // int64
-    size_t evox, eslot, list, hkey, hmin, hu, uvox, uslot, ucnt, uoff;   // int32
+    size_t evox, eslot, list, hkey, hmin, hu, uvox, uslot, ucnt, uoff, spk;   // int32
     size_t stold, stnew;                           // int8
     size_t total;
     int cap, ecap, H, hbits;
@@ -51,7 +51,7 @@ __host__ __device__ inline ArenaLayout arena_layout(int cap, int fcap, int Bp1, 
     L.bnew = o;  o += sizeof(double) * 9 * (size_t)cap;
     L.tanT = o;  o += sizeof(double) * 9 * (size_t)cap;
     L.epart = o; o += sizeof(double) * (size_t)cap;
-    L.pw = o;    o += sizeof(double) * 3 * 2 * (size_t)cap;
+    L.c0 = o;    o += sizeof(double) * (size_t)L.ecap;
     L.udrho = o; o += sizeof(double) * (size_t)Bp1 * L.ecap;
     L.urho = o;  o += sizeof(double) * (size_t)Bp1 * L.ecap;
     L.inds = o;  o += sizeof(int64_t) * (size_t)cap;
@@ -65,6 +65,7 @@ __host__ __device__ inline ArenaLayout arena_layout(int cap, int fcap, int Bp1, 
     L.uslot = o; o += sizeof(int32_t) * (size_t)L.ecap;
     L.ucnt = o;  o += sizeof(int32_t) * (size_t)L.ecap;
     L.uoff = o;  o += sizeof(int32_t) * (size_t)(L.ecap + 1);
+    L.spk = o;   o += sizeof(int32_t) * 2 * (size_t)cap;
     L.stold = o; o += (size_t)CHROMO_MAX_BINDERS * cap;
     L.stnew = o; o += (size_t)CHROMO_MAX_BINDERS * cap;
     L.total = align_up(o, 256);
@@ -90,9 +91,9 @@ __host__ __device__ inline ArenaLayout global_layout(int64_t bead_cap, int64_t s
 }
 
 struct Arena {
-    double *bnew, *tanT, *epart, *pw, *udrho, *urho;
+    double *bnew, *tanT, *epart, *c0, *udrho, *urho;
     int64_t *inds;
-    int32_t *evox, *eslot, *list, *hkey, *hmin, *hu, *uvox, *uslot, *ucnt, *uoff;
+    int32_t *evox, *eslot, *list, *hkey, *hmin, *hu, *uvox, *uslot, *ucnt, *uoff, *spk;
     int8_t *stold, *stnew;
     int cap, ecap, hmask, hshift;
 };
@@ -103,7 +104,7 @@ __device__ inline Arena make_arena(char *base, const ArenaLayout &L)
     A.bnew = (double *)(base + L.bnew);
     A.tanT = (double *)(base + L.tanT);
     A.epart = (double *)(base + L.epart);
-    A.pw = (double *)(base + L.pw);
+    A.c0 = (double *)(base + L.c0);
     A.udrho = (double *)(base + L.udrho);
     A.urho = (double *)(base + L.urho);
     A.inds = (int64_t *)(base + L.inds);
@@ -117,6 +118,7 @@ __device__ inline Arena make_arena(char *base, const ArenaLayout &L)
     A.uslot = (int32_t *)(base + L.uslot);
     A.ucnt = (int32_t *)(base + L.ucnt);
     A.uoff = (int32_t *)(base + L.uoff);
+    A.spk = (int32_t *)(base + L.spk);
     A.stold = (int8_t *)(base + L.stold);
     A.stnew = (int8_t *)(base + L.stnew);
     A.cap = L.cap;
@@ -306,9 +308,19 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, WaveRng &rng,
                 double wt[3];
                 voxel_lower(R, pos, ix, wt);
                 const int p = (int)(2 * i + k);
-                for (int j = 0; j < 3; ++j) A.pw[j * 2 * cap + p] = wt[j];
+                const double vol_bin = R->vol_bin;
+                // binder states of this configuration, one byte per binder
+                int32_t sp = 0;
 #pragma unroll
-                for (int l = 0; l < 8; ++l) A.evox[p * 8 + l] = (int32_t)corner_voxel(R, ix, l);
+                for (int m = 0; m < NB; ++m)
+                    sp |= ((int32_t)(uint8_t)((k == 0 || !state_change) ? so[m] : sn[m])) << (8 * m);
+                A.spk[p] = sp;
+#pragma unroll
+                for (int l = 0; l < 8; ++l) {
+                    A.evox[p * 8 + l] = (int32_t)corner_voxel(R, ix, l);
+                    // weight / access volume of this contribution (fields.pyx:1472-1488)
+                    A.c0[p * 8 + l] = corner_weight(l, wt[0], wt[1], wt[2]) / vol_bin;
+                }
             }
         }
     }
@@ -378,13 +390,17 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, WaveRng &rng,
             for (int q = q0; q < q1; ++q) {
                 A.uoff[q] = off;
                 off += A.ucnt[q];
+                A.ucnt[q] = 0;
             }
             if (lane == 0) A.uoff[U] = E;
         }
         arena_sync<KIND>();
+        // counting-sort placement, filling each voxel's list from the front so that
+        // it comes out (nearly) in contribution order; the insertion sort below
+        // makes it exact
         for (int e = lane; e < E; e += WAVE) {
             int q = A.hu[A.eslot[e]];
-            int pos = A.uoff[q] + atomicSub(&A.ucnt[q], 1) - 1;
+            int pos = A.uoff[q] + atomicAdd(&A.ucnt[q], 1);
             A.list[pos] = e;
         }
         arena_sync<KIND>();
@@ -394,7 +410,6 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, WaveRng &rng,
         double S2[NB], SX[NB * NB], NS = 0;
         for (int j = 0; j < NB; ++j) S2[j] = 0;
         for (int j = 0; j < NB * NB; ++j) SX[j] = 0;
-        const double vol_bin = R->vol_bin;
         for (int e = lane; e < E; e += WAVE) {
             const int hs = A.eslot[e];
             if (A.hmin[hs] != e) continue;
@@ -409,22 +424,35 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, WaveRng &rng,
                 }
                 A.list[c + 1] = x;
             }
-            double acc[CHROMO_MAX_BINDERS + 1];
-            for (int m = 0; m < Bp1; ++m) acc[m] = 0;
-            for (int a = s0; a < s1; ++a) {
-                const int e = A.list[a];
-                const int p = e >> 3, l = e & 7;
-                const int i = p >> 1, k = p & 1;
-                const double wt = corner_weight(l, A.pw[p], A.pw[2 * cap + p], A.pw[4 * cap + p]);
-                const double c0 = wt / vol_bin;
-                const double pre = k ? 1. : -1.;
-                const double t0 = pre * c0;
-                if (fabs(t0) > 1E-18) acc[0] += t0;
-                for (int m = 1; m < Bp1; ++m) {
-                    int st = (k == 0 || !state_change) ? A.stold[(m - 1) * cap + i]
-                                                       : A.stnew[(m - 1) * cap + i];
-                    double t = pre * (c0 * (double)st);
-                    if (fabs(t) > 1E-18) acc[m] += t;
+            double acc[NB + 1];
+#pragma unroll
+            for (int m = 0; m <= NB; ++m) acc[m] = 0;
+            // items in groups of four: their loads are issued together, the adds
+            // then run in list order
+            for (int a = s0; a < s1; a += 4) {
+                int ev[4];
+                double cv[4];
+                int32_t sv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) ev[u] = (a + u < s1) ? A.list[a + u] : 0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    cv[u] = (a + u < s1) ? A.c0[ev[u]] : 0.0;
+                    sv[u] = (a + u < s1) ? A.spk[ev[u] >> 3] : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (a + u >= s1) break;
+                    const double c0 = cv[u];
+                    const double pre = ((ev[u] >> 3) & 1) ? 1. : -1.;
+                    const double t0 = pre * c0;
+                    if (fabs(t0) > 1E-18) acc[0] += t0;
+#pragma unroll
+                    for (int m = 1; m <= NB; ++m) {
+                        const int st = (sv[u] >> (8 * (m - 1))) & 0xff;
+                        const double t = pre * (c0 * (double)st);
+                        if (fabs(t) > 1E-18) acc[m] += t;
+                    }
                 }
             }
             double r[CHROMO_MAX_BINDERS + 1];
@@ -722,7 +750,7 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
     if (n == 0) return;   // mc_sim.pyx:149-151: no dE, no Metropolis draw
     if (P.move == CHROMO_TANGENT_ROTATION) {
         // report the first picked bead
-        out.ind0 = __shfl(P.mypick, 0, WAVE);
+        out.ind0 = readlane_i64(P.mypick, 0);
         out.indf = out.ind0 + 1;
     }
     if (n > SEG_LDS) {

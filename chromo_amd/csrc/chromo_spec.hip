@@ -36,9 +36,14 @@ hipError_t launch_nb(const chromo_replica *d_reps, int n_reps, int64_t steps, hi
     constexpr int W = CHROMO_SPEC_W;
     const size_t bytes = spec_layout(NB + 1, W).total;
     if (bytes > (size_t)lim) return hipErrorInvalidValue;
-    hipError_t e = hipFuncSetAttribute((const void *)k_mc_spec<W, NB>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) return e;
+    // the attribute is set once per process: setting it costs far more than a launch
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_mc_spec<W, NB>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
     hipLaunchKernelGGL((k_mc_spec<W, NB>), dim3(n_reps), dim3(WAVE * W), bytes, st, d_reps, n_reps, steps);
     return hipGetLastError();
 }
@@ -53,11 +58,14 @@ namespace chromo {
 hipError_t CHROMO_CAT(spec_launch_w, CHROMO_SPEC_W)(const chromo_replica *d_reps, int n_reps, int64_t steps,
                                                     int binder_mask, hipStream_t st)
 {
-    int dev = 0, lim = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return e;
-    e = hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    if (e != hipSuccess) return e;
+    static int lim = 0;
+    hipError_t e = hipSuccess;
+    if (lim == 0) {
+        int dev = 0;
+        if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+        if ((e = hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)) != hipSuccess)
+            return e;
+    }
     if ((binder_mask & 2) && (e = launch_nb<1>(d_reps, n_reps, steps, st, lim)) != hipSuccess) return e;
     if ((binder_mask & 4) && (e = launch_nb<2>(d_reps, n_reps, steps, st, lim)) != hipSuccess) return e;
     if ((binder_mask & 8) && (e = launch_nb<3>(d_reps, n_reps, steps, st, lim)) != hipSuccess) return e;

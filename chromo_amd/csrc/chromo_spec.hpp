@@ -112,18 +112,6 @@ struct SpecKeep {
     int own[2], ins[2];
 };
 
-__device__ __forceinline__ int64_t readlane_i64(int64_t v, int l)
-{
-    uint32_t lo = readlane_u32((uint32_t)v, l);
-    uint32_t hi = readlane_u32((uint32_t)((uint64_t)v >> 32), l);
-    return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-
-__device__ __forceinline__ double readlane_f64(double v, int l)
-{
-    return __longlong_as_double((long long)readlane_i64((int64_t)__double_as_longlong(v), l));
-}
-
 __device__ __forceinline__ void rng_save(const WaveRng &g, uint32_t *ck)
 {
     const int lane = lane_id();
@@ -515,7 +503,7 @@ __device__ __forceinline__ void eval_small(const chromo_replica *R, const SpecSl
             for (int m = 0; m < NB; ++m) mods[m] = R->marks[(int64_t)m * N + b];
             e = bead_binding_dE<NB>(R, mods, so, sn);
         }
-        for (int i = 0; i < n; ++i) dEp += __shfl(e, i, WAVE);
+        for (int i = 0; i < n; ++i) dEp += readlane_f64(e, i);
     } else {
         double ef = 0, er = 0;
         if (lane == 0 && ind0 != 0) {
@@ -536,8 +524,8 @@ __device__ __forceinline__ void eval_small(const chromo_replica *R, const SpecSl
             Bond bd = load_bond(R->bonds, N, indf - 1);
             er = pair_dE_reverse(bd, &o[0], &w[0], r1, &o[3], &w[3], t31);
         }
-        if (ind0 != 0) dEp += __shfl(ef, 0, WAVE);
-        if (indf != N) dEp += __shfl(er, (int)(n - 1), WAVE);
+        if (ind0 != 0) dEp += readlane_f64(ef, 0);
+        if (indf != N) dEp += readlane_f64(er, (int)(n - 1));
     }
 
     // field dE (UniformDensityField.compute_dE, fields.pyx:1149-1233)
@@ -700,12 +688,6 @@ __device__ __forceinline__ void eval_small(const chromo_replica *R, const SpecSl
     }
 }
 
-__device__ __forceinline__ uint32_t wave_or(uint32_t m)
-{
-    for (int d = 32; d >= 1; d >>= 1) m |= (uint32_t)__shfl_xor((int)m, d, WAVE);
-    return m;
-}
-
 // Earlier slots of this round whose writes slot s reads.
 __device__ __forceinline__ void spec_conflicts(const SpecSlot *slots, SpecRes *res, const SpecKeep &K, const SpecLds &L,
                                int s)
@@ -828,12 +810,8 @@ __device__ __forceinline__ void spec_book(const chromo_replica *R, chromo_ctrl &
     const uint64_t live_m = __ballot(live);
     const uint64_t acc_m = __ballot(live && acc);
     const uint64_t tie_m = __ballot(live && tie);
-    int64_t sn = n, sb = nb, sy = bytes;
-    for (int d = SPEC_WMAX / 2; d >= 1; d >>= 1) {
-        sn += __shfl_xor(sn, d, WAVE);
-        sb += __shfl_xor(sb, d, WAVE);
-        sy += __shfl_xor(sy, d, WAVE);
-    }
+    static_assert(SPEC_WMAX <= 8, "bookkeeping sums over the first eight lanes");
+    const int64_t sn = sum_lanes8_i64(n), sb = sum_lanes8_i64(nb), sy = sum_lanes8_i64(bytes);
     const int64_t logged = K.logged;
     const int nlive = __popcll(live_m);
     if (R->log != nullptr && live) {

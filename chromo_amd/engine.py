@@ -149,7 +149,7 @@ class DeviceReplicas:
     """Chains, fields, controllers and RNG streams of ``n`` replicas in HBM."""
 
     CTRL_SLOTS = 8
-    WAVE_CHOICES = (1, 2, 4)
+    WAVE_CHOICES = (1, 2)
     waves = 1          # wavefronts per replica used by run(); see chromo_amd.autotune
 
     def __init__(self, chains: Sequence[ChainSpec], fields: Sequence[FieldSpec],
@@ -352,6 +352,22 @@ class DeviceReplicas:
                                              _current_stream()), "chromo_mc_run_tuned")
         if sync:
             self.raise_on_error()
+
+    def capture(self, num_steps: int = 1, *, waves: Optional[int] = None) -> "torch.cuda.CUDAGraph":
+        """A HIP graph of ``run(num_steps)``: ``g.replay()`` advances every replica by
+        num_steps sweeps with one graph launch (the launches record no memsets)."""
+        w = int(self.waves if waves is None else waves)
+        self.run(0, waves=w)             # sets the launch attributes outside the capture
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                L.check(self.lib.chromo_mc_run_tuned(self.tab_ptr, self.n, int(num_steps), w,
+                                                     self.binder_mask, _current_stream()),
+                        "chromo_mc_run_tuned (capture)")
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        return g
 
     def run_plain(self, num_steps: int, rep: int = 0, *, sync: bool = True) -> None:
         """The same schedule with one kernel launch per proposal."""

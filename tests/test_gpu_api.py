@@ -91,3 +91,33 @@ def test_plain_launch_matches_persistent():
         res.append((p.r.copy(), p.states.copy(), udf.density.copy()))
     for a, b in zip(res[0], res[1]):
         assert np.array_equal(a, b)
+
+
+def test_graph_replays_match_eager():
+    """A captured sweep replayed several times (with eager work and syncs in
+    between) gives the same chains as the same number of eager sweeps."""
+    from chromo_amd import _lib as L
+    from chromo_amd import workloads as W
+    from chromo_amd.engine import DeviceReplicas
+    import torch
+    print("graph replay vs eager", flush=True)
+    systems = [W.make_system(n, 2, g, seed=60 + i) for i, (n, g) in enumerate([(2500, 14), (900, 10)])]
+
+    def make():
+        return DeviceReplicas([s.chain for s in systems], [s.field for s in systems],
+                              [copy.deepcopy(s.ctrls) for s in systems], rng_mode=L.RNG_PHILOX,
+                              seeds=[3, 3], streams=[0, 1])
+    for w in (1, 2):
+        eager, graphed = make(), make()
+        g = graphed.capture(1, waves=w)
+        for rep in range(4):
+            eager.run(1, waves=w)
+            g.replay()
+            torch.cuda.synchronize()
+            _ = torch.ones(1000, device="cuda").sum().item()      # eager work between replays
+            for i in range(2):
+                for a, b in zip(eager.positions(i), graphed.positions(i)):
+                    assert np.array_equal(a, b), (w, rep, i)
+                assert np.array_equal(eager.density_of(i), graphed.density_of(i))
+                assert np.array_equal(eager.rng_words(i), graphed.rng_words(i))
+        graphed.raise_on_error()

@@ -17,9 +17,13 @@ from chromo_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-# Eight wavefronts per replica: a benchmark-size run of that build faulted the GPU
-# (out-of-aperture access) and the cause is not found; the entry point rejects it.
+# Eight and four wavefronts per replica: those builds spill heavily; a benchmark-size
+# run of the 8-wavefront build faulted the GPU (out-of-aperture access), and with 4
+# wavefronts one build gave wrong voxel counts for three binder types and another
+# hung.  The cause is not found; the entry point rejects both.
 W8_SKIP = "waves_per_replica = 8 disabled after a GPU fault with no cause found yet"
+W4_SKIP = "waves_per_replica = 4 disabled after a GPU hang and wrong results with no cause found yet"
+P4 = pytest.param(4, marks=pytest.mark.skip(reason=W4_SKIP))
 
 
 def say(*a):
@@ -154,7 +158,7 @@ def _compare_run(oracle, s, steps, inner_seed=123, rng_mode=0, seed=1, stream=0,
     return lo
 
 
-@pytest.mark.parametrize("waves", [1, 4])
+@pytest.mark.parametrize("waves", [1, 2, P4])
 @pytest.mark.parametrize("N,nvox,seed", [(300, 8, 11), (1000, 30, 12), (5000, 20, 13)])
 def test_mc_sim_glibc_parity(oracle, N, nvox, seed, waves):
     say("mc_sim glibc", N, nvox, seed)
@@ -170,14 +174,15 @@ def test_mc_sim_philox_parity(oracle, waves):
     _compare_run(oracle, s, steps=12, rng_mode=1, seed=777, stream=5, waves=waves)
 
 
-@pytest.mark.parametrize("waves", [1, 4])
+@pytest.mark.parametrize("waves", [1, 2, P4])
 def test_mc_sim_three_binders_and_cross_talk(oracle, waves):
     say("mc_sim B=3")
     s = W.make_system(600, 3, 10, seed=22, eps=[-4, -3, -2], cross_eps=0.5, mu=[-0.4, 0.2, -1])
     _compare_run(oracle, s, steps=10, waves=waves)
 
 
-@pytest.mark.parametrize("B,waves", [(1, 4), (4, 4), pytest.param(4, 8, marks=pytest.mark.skip(reason=W8_SKIP))])
+@pytest.mark.parametrize("B,waves", [(1, 2), (4, 2), pytest.param(1, 4, marks=pytest.mark.skip(reason=W4_SKIP)),
+                                     pytest.param(4, 8, marks=pytest.mark.skip(reason=W8_SKIP))])
 def test_mc_sim_one_and_four_binders(oracle, B, waves):
     say("mc_sim B", B)
     s = W.make_system(700, B, 10, seed=40 + B, eps=-3.0, cross_eps=0.3 if B > 1 else 0.0,
@@ -185,7 +190,7 @@ def test_mc_sim_one_and_four_binders(oracle, B, waves):
     _compare_run(oracle, s, steps=8, waves=waves)
 
 
-@pytest.mark.parametrize("waves", [1, 4])
+@pytest.mark.parametrize("waves", [1, 2, P4])
 def test_mc_sim_large_moves_use_global_arena(oracle, waves):
     """Crank and slide windows up to 150 beads: moves above 32 beads run through
     the global scratch arena (and, with several wavefronts, one at a time)."""
@@ -199,7 +204,7 @@ def test_mc_sim_large_moves_use_global_arena(oracle, waves):
     assert lo["n"].max() > 32
 
 
-@pytest.mark.parametrize("waves", [2, 4, pytest.param(8, marks=pytest.mark.skip(reason=W8_SKIP))])
+@pytest.mark.parametrize("waves", [2, P4, pytest.param(8, marks=pytest.mark.skip(reason=W8_SKIP))])
 def test_mc_sim_short_chain_conflicts(oracle, waves):
     """A 40-bead chain on a 3^3 grid: proposals of one round overlap in beads and
     voxels all the time, so most rounds re-evaluate."""
@@ -209,7 +214,8 @@ def test_mc_sim_short_chain_conflicts(oracle, waves):
     assert lo["accepted"].sum() > 50
 
 
-@pytest.mark.parametrize("nvox,waves", [(1, 4), (2, 4), (1, 1)])
+@pytest.mark.parametrize("nvox,waves", [(1, 2), (2, 2), (1, 1),
+                                        pytest.param(1, 4, marks=pytest.mark.skip(reason=W4_SKIP))])
 def test_mc_sim_tiny_grids(oracle, nvox, waves):
     """Grids of one or two voxels per side: corners wrap onto the same voxel."""
     say("mc_sim tiny grid", nvox)
@@ -218,7 +224,9 @@ def test_mc_sim_tiny_grids(oracle, nvox, waves):
     _compare_run(oracle, s, steps=6, waves=waves)
 
 
-@pytest.mark.parametrize("waves,rng_mode", [(1, 0), (4, 0), (2, 1), pytest.param(8, 1, marks=pytest.mark.skip(reason=W8_SKIP))])
+@pytest.mark.parametrize("waves,rng_mode", [(1, 0), (2, 0), (2, 1),
+                                            pytest.param(4, 0, marks=pytest.mark.skip(reason=W4_SKIP)),
+                                            pytest.param(8, 1, marks=pytest.mark.skip(reason=W8_SKIP))])
 def test_mc_sim_coincident_beads_redraw_crank_axis(oracle, waves, rng_mode):
     """Beads 200..260 sit on one point, so crank-shaft axes there are shorter
     than 1e-5 and the reference draws a random axis (two extra draws); with
@@ -241,7 +249,7 @@ def test_all_wave_counts_identical():
     systems = [W.make_system(n, 2, g, seed=50 + i, chi=0.4 + 0.3 * i)
                for i, (n, g) in enumerate([(3000, 16), (800, 10), (12000, 24)])]
     runs = []
-    for w in (1, 2, 4):   # 8: disabled, see W8_SKIP
+    for w in (1, 2):   # 4 and 8: disabled, see W4_SKIP / W8_SKIP
         dr = _dev(systems, rng_mode=1, seeds=[9] * 3, streams=[0, 1, 2], log_cap=20000)
         dr.run(12, waves=w)
         runs.append(dr)
