@@ -36,6 +36,8 @@ ERROR_TEXT = {
     2: "move touches more beads than the replica's scratch arena holds (seg_cap)",
     3: "tangent rotation asked for more distinct beads than the polymer has",
     4: "binder count outside 1..4",
+    5: "a rejection-sampling or probe loop exceeded its iteration bound (amplitude below 1?)",
+    6: "internal move-arena index check failed",
 }
 
 

@@ -37,7 +37,14 @@ enum : int {
     ERR_SEGCAP = 2,       // move larger than the scratch arena
     ERR_TANGENT_K = 3,    // more distinct beads requested than the chain has
     ERR_NBINDERS = 4,     // binding move on a chain without binders
+    ERR_SPIN = 5,         // a rejection or probe loop ran past SPIN_LIMIT iterations
+    ERR_ARENA = 6,        // a move-arena index check failed (internal error)
 };
+
+// Iteration bound of every data-dependent loop (rejection sampling, distinct-bead
+// picks, hash probes).  Valid inputs stay far below it; a loop that reaches it
+// stops with ERR_SPIN, so every wavefront always reaches the end of the kernel.
+constexpr int SPIN_LIMIT = 1 << 20;
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
 // Wavefront index in the workgroup, as a wave-uniform (scalar) value.
@@ -386,12 +393,23 @@ __device__ __forceinline__ int64_t imax64(int64_t a, int64_t b) { return a > b ?
 __device__ __forceinline__ int64_t imin64(int64_t a, int64_t b) { return a < b ? a : b; }
 
 // capped_exponential, bead_selection.pyx:19-66
-__device__ __forceinline__ int64_t capped_exponential(WaveRng &R, int64_t window, int64_t cap)
+// The draw accepts r <= cap with probability above 0.99 for cap >= 1; cap < 1 can
+// never accept (the reference loops for ever) and is reported as ERR_SPIN.
+__device__ __forceinline__ int64_t capped_exponential(WaveRng &R, int64_t window, int64_t cap, int &err)
 {
+    if (cap < 1) {
+        err = ERR_SPIN;
+        return 0;
+    }
     int64_t r;
+    int it = 0;
     do {
         r = (int64_t)(-log10(((double)rnd31(R) / RMAX) + 0.00001) * (double)window * 0.45 +
                       1.0001);
+        if (++it >= SPIN_LIMIT) {
+            err = ERR_SPIN;
+            return 0;
+        }
     } while (r > cap);
     return r;
 }
@@ -402,7 +420,7 @@ __device__ __forceinline__ int64_t from_left(WaveRng &R, int64_t w, int64_t nb, 
         err = ERR_WINDOW;
         return 0;
     }
-    return capped_exponential(R, w, w);
+    return capped_exponential(R, w, w, err);
 }
 
 __device__ __forceinline__ int64_t from_right(WaveRng &R, int64_t w, int64_t nb, int &err)

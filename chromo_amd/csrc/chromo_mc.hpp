@@ -340,17 +340,30 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, WaveRng &rng,
     int U = 0;
     if (check_field) {
         const int E = (int)(16 * n);
-        // unique voxels: hash insert, first-occurrence index per voxel
+        // unique voxels: hash insert, first-occurrence index per voxel.  The table
+        // holds at least twice the contributions, so a probe that visits every
+        // slot means it was not cleared: ERR_ARENA.
+        int bad = 0;
         for (int e = lane; e < E; e += WAVE) {
             int v = A.evox[e];
             uint32_t h = ((uint32_t)v * 2654435761u) >> A.hshift;
+            int probes = 0;
             for (;;) {
                 int old = atomicCAS(&A.hkey[h], HKEY_EMPTY, v);
                 if (old == HKEY_EMPTY || old == v) break;
                 h = (h + 1) & (uint32_t)A.hmask;
+                if (++probes > A.hmask) {
+                    bad = 1;
+                    break;
+                }
             }
+            if (bad) break;
             atomicMin(&A.hmin[h], e);
             A.eslot[e] = (int)h;
+        }
+        if (__any(bad)) {
+            out.err = ERR_ARENA;
+            return;
         }
         arena_sync<KIND>();
         // number unique voxels in first-occurrence order
@@ -361,6 +374,10 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, WaveRng &rng,
         for (int e = e0; e < e1; ++e)
             if (A.hmin[A.eslot[e]] == e) ++cnt;
         int base = wave_excl_scan(cnt, U);
+        if (U < 1 || U > E) {
+            out.err = ERR_ARENA;
+            return;
+        }
         for (int e = e0; e < e1; ++e) {
             int s = A.eslot[e];
             if (A.hmin[s] == e) {
@@ -398,10 +415,16 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, WaveRng &rng,
         // counting-sort placement, filling each voxel's list from the front so that
         // it comes out (nearly) in contribution order; the insertion sort below
         // makes it exact
+        int outside = 0;
         for (int e = lane; e < E; e += WAVE) {
             int q = A.hu[A.eslot[e]];
-            int pos = A.uoff[q] + atomicAdd(&A.ucnt[q], 1);
-            A.list[pos] = e;
+            int pos = (q >= 0 && q < U) ? A.uoff[q] + atomicAdd(&A.ucnt[q], 1) : -1;
+            if (pos >= 0 && pos < E) A.list[pos] = e;
+            else outside = 1;
+        }
+        if (__any(outside)) {
+            out.err = ERR_ARENA;
+            return;
         }
         arena_sync<KIND>();
         // per voxel: order contributions, fold (fields.pyx:1491-1520), energy terms.
@@ -676,8 +699,12 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
         Arena A;
         if (k <= SEG_LDS) A = make_arena(lds_dyn, lds_layout(NB + 1));
         else A = make_arena((char *)R->scratch, global_layout(R->bead_cap, R->seg_cap, NB + 1));
-        for (int64_t i = 0; i < k; ++i) {
-            for (;;) {
+        for (int64_t i = 0; i < k && !err; ++i) {
+            for (int it = 0;; ++it) {
+                if (it >= SPIN_LIMIT) {
+                    err = ERR_SPIN;
+                    break;
+                }
                 int64_t c = rnd31(rng) % N;
                 int dup = (lane < i && P.mypick == c);
                 for (int64_t j = WAVE + lane; j < i; j += WAVE) dup |= (A.inds[j] == c);
@@ -692,6 +719,7 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
                 }
             }
         }
+        if (err) goto fail;
         const double origin[3] = {0., 0., 0.};
         for (int64_t i = 0; i < k; ++i) {
             double dir[3], T[12];
@@ -762,6 +790,10 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
     }
     if (n <= SEG_LDS) move_body<0, NB>(R, rng, P, out);
     else move_body<1, NB>(R, rng, P, out);
+    if (out.err) {
+        out.n = 0;
+        return;
+    }
     // AcceptanceTracker.update_acceptance_rate (mc_stat.py:190-208)
     if (out.accepted) {
         C.num_success += 1;

@@ -215,8 +215,13 @@ __device__ __forceinline__ int gen_slot(const chromo_replica *R, WaveRng &rng, c
         seq = (k > N) || (k > SPEC_NMAX);
         if (!seq) {
             int64_t mine = 0;
-            for (int64_t i = 0; i < k; ++i) {
-                for (;;) {
+            for (int64_t i = 0; i < k && !seq; ++i) {
+                for (int it = 0;; ++it) {
+                    // a pick that does not end is left to mc_step_dev, which reports it
+                    if (it >= SPIN_LIMIT) {
+                        seq = true;
+                        break;
+                    }
                     int64_t c = rnd31(rng) % N;
                     if (!__any(lane < i && mine == c)) {
                         if (lane == i) mine = c;
@@ -225,7 +230,7 @@ __device__ __forceinline__ int gen_slot(const chromo_replica *R, WaveRng &rng, c
                 }
             }
             if (lane < k) S.picks[lane] = mine;
-            for (int64_t i = 0; i < k; ++i) {
+            for (int64_t i = 0; i < k && !seq; ++i) {
                 int32_t a = rnd31(rng);
                 int32_t b = rnd31(rng);
                 if (lane == 0) {
@@ -657,10 +662,20 @@ __device__ __forceinline__ void eval_small(const chromo_replica *R, const SpecSl
             for (int j = 0; j < 2; ++j) {
                 if (!K.own[j]) continue;
                 uint32_t h = ((uint32_t)vox[j] * 2654435761u) >> SPEC_HASH_SHIFT;
+                int probes = 0;
+                bool full = false;
                 for (;;) {
                     int old = atomicCAS(&L.hkey[h], HKEY_EMPTY, vox[j]);
                     if (old == HKEY_EMPTY || old == vox[j]) break;
                     h = (h + 1) & (uint32_t)(SPEC_HASH - 1);
+                    if (++probes >= SPEC_HASH) {
+                        full = true;
+                        break;
+                    }
+                }
+                if (full) {
+                    res.overflow = 1;
+                    continue;
                 }
                 atomicOr(&L.hmask[h], 1u << slot);
                 K.hs[j] = (int32_t)h;
@@ -862,10 +877,20 @@ __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng
     int cur = 0;
     int nq = 0;                        // wavefront 0: slots queued in the current buffer
     int remaining = C.num_per_cycle;   // wavefront 0: proposals of the block not yet resolved
+    int pend = 0;                      // wavefront 0: error to publish before the next barrier
+    int64_t rounds = 0;
+    // every round resolves a proposal or turns its first slot into a sequential one
+    const int64_t max_rounds = 2 * (int64_t)C.num_per_cycle + 64;
     for (;;) {
         SpecSlot *cs = cur ? L.slots[1] : L.slots[0];
         SpecSlot *ns = cur ? L.slots[0] : L.slots[1];
-        if (wv == 0 && !L.ctl->err) spec_fill<NB, W>(R, rng, C, L, cs, nq, remaining);
+        if (wv == 0) {
+            // L.ctl->err is written only here, before the barrier after which every
+            // wavefront reads it
+            if (++rounds > max_rounds) pend = ERR_SPIN;
+            if (pend && lane == 0) L.ctl->err = pend;
+            if (!pend && !L.ctl->err) spec_fill<NB, W>(R, rng, C, L, cs, nq, remaining);
+        }
         __syncthreads();
         if (L.ctl->err || L.ctl->remaining <= 0) break;
         const int cnt = L.ctl->cnt, nqa = L.ctl->nq;
@@ -885,7 +910,7 @@ __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng
                 wave_sync();
                 const MoveOut o = io->out;
                 if (o.err) {
-                    if (lane == 0) L.ctl->err = o.err;
+                    pend = o.err;
                 } else if (o.n > 0) {
                     record(R, C, o, K);
                 }
@@ -905,11 +930,13 @@ __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng
             for (int s = 0; s < cnt; ++s) {
                 if (cs[s].kind == SLOT_EMPTY) continue;
                 const SpecRes &q = L.res[s];
-                if ((q.cmask & accm) || q.overflow) {
+                if ((q.cmask & accm) || (q.overflow && s > 0)) {
                     F = s;
                     break;
                 }
-                if (q.resample) {
+                // the first slot always fits the conflict table; if it did not, the
+                // sequential path redoes it
+                if (q.resample || q.overflow) {
                     F = s;
                     rew = 1;
                     break;

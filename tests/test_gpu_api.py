@@ -121,3 +121,32 @@ def test_graph_replays_match_eager():
                 assert np.array_equal(eager.density_of(i), graphed.density_of(i))
                 assert np.array_equal(eager.rng_words(i), graphed.rng_words(i))
         graphed.raise_on_error()
+
+
+@pytest.mark.parametrize("waves", [1, 2])
+def test_unsatisfiable_draw_stops_the_replica(waves):
+    """An end-pivot amplitude of 0 beads makes the capped exponential draw
+    unsatisfiable (bead_selection.pyx:19-66 would loop for ever).  The kernels stop
+    that replica with error 5 at its first end-pivot proposal and keep the other
+    replica running, so the launch always completes."""
+    from chromo_amd import _lib as L
+    from chromo_amd import workloads as W
+    from chromo_amd.engine import DeviceReplicas
+    import torch
+    say("unsatisfiable draw, waves", waves)
+    systems = [W.make_system(600, 2, 8, seed=70), W.make_system(600, 2, 8, seed=71)]
+    ctrls = [copy.deepcopy(s.ctrls) for s in systems]
+    for c in ctrls[0]:
+        if c.move == L.END_PIVOT:
+            c.amp_bead, c.bead_lo, c.controlled = 0, 0, 0
+    dr = DeviceReplicas([s.chain for s in systems], [s.field for s in systems], ctrls,
+                        rng_mode=L.RNG_PHILOX, seeds=[5, 5], streams=[0, 1])
+    dr.run(3, waves=waves, sync=False)
+    torch.cuda.synchronize()
+    cnt = dr.counter_array()
+    assert cnt[0, L.CNT_ERROR] == 5
+    # the first sweep's crank-shaft block (30 proposals) ran before the end pivot
+    assert 0 < cnt[0, L.CNT_PROPOSALS] <= 30
+    assert cnt[1, L.CNT_ERROR] == 0 and cnt[1, L.CNT_PROPOSALS] > 3 * 100
+    with pytest.raises(L.ChromoError, match="iteration bound"):
+        dr.raise_on_error()
