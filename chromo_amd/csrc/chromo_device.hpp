@@ -89,6 +89,15 @@ __device__ __forceinline__ double readlane_f64(double v, int l)
 }
 
 __device__ __forceinline__ double bcast0(double x) { return readlane_f64(x, 0); }
+
+// A value every lane holds equally, marked as such for the compiler (scalar
+// registers and scalar arithmetic).
+__device__ __forceinline__ uint32_t uni_u32(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
+__device__ __forceinline__ int uni_i32(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t uni_u64(uint64_t x)
+{
+    return ((uint64_t)uni_u32((uint32_t)(x >> 32)) << 32) | uni_u32((uint32_t)x);
+}
 __device__ __forceinline__ int64_t bcast0(int64_t x) { return readlane_i64(x, 0); }
 __device__ __forceinline__ int bcast0(int x) { return (int)readlane_u32((uint32_t)x, 0); }
 
@@ -287,7 +296,7 @@ __device__ __forceinline__ uint32_t mt_draw(WaveRng &R)
         mt_twist(R.mt);
         R.mti = 0;
     }
-    uint32_t y = R.mt[R.mti++];
+    uint32_t y = uni_u32(R.mt[R.mti++]);
     y ^= y >> 11;
     y ^= (y << 7) & 0x9d2c5680u;
     y ^= (y << 15) & 0xefc60000u;
@@ -302,6 +311,13 @@ __device__ __forceinline__ int32_t rnd31(WaveRng &R)
 }
 
 __device__ __forceinline__ double rnd_unit(WaveRng &R) { return (double)rnd31(R) / RMAX; }
+
+// C's x % m for a draw x in [0, 2^31) and m >= 1, in 32-bit arithmetic.
+__device__ __forceinline__ int64_t draw_mod(int32_t x, int64_t m)
+{
+    if (m > 0x7fffffff) return x;
+    return (int64_t)((uint32_t)x % (uint32_t)m);
+}
 
 // numpy legacy RandomState.randint(0, high): masked rejection over 32-bit draws
 // (chromo/mc/move_funcs.pyx:819).
@@ -378,9 +394,16 @@ __device__ __forceinline__ double rowdot(const double *Trow, double x0, double x
 }
 
 // Cython float modulo with cdivision=False (fmod + sign fix).
+// Python's float a % b for b > 0: fmod (exact), then b added to a nonzero result of
+// the other sign.  For |a| < 2b fmod's exact result is a, a - b or a + b, and
+// those differences are exact (Sterbenz), so no library fmod is needed there.
 __device__ __forceinline__ double pymod(double a, double b)
 {
-    double r = fmod(a, b);
+    double r;
+    if (b > 0 && fabs(a) < b) r = a;
+    else if (b > 0 && a >= b && a < 2 * b) r = a - b;
+    else if (b > 0 && a <= -b && a > -2 * b) r = copysign(a + b, a);   // fmod keeps a's sign, also for 0
+    else r = fmod(a, b);
     r += (double)((r != 0) & ((r < 0) ^ (b < 0))) * b;
     return r;
 }
@@ -436,7 +459,7 @@ __device__ __forceinline__ int64_t from_point(WaveRng &R, int64_t w, int64_t nb,
         return 0;
     }
     if (w < 1) return ind0;
-    int64_t side = rnd31(R) % 2;
+    int64_t side = rnd31(R) & 1;
     if (side == 0) {
         int64_t ws = imax64(imin64(w, ind0), 1);
         int64_t ub = imax64(ind0, 1);

@@ -36,9 +36,9 @@ __global__ void __launch_bounds__(64) k_mc_run(const chromo_replica *reps, int n
         return;
     }
     const int Bp1 = R->B + 1;
-    ArenaLayout L = lds_layout(Bp1);
-    uint32_t *mt_lds = (uint32_t *)(lds_dyn + L.total);
-    lds_hash_clear(Bp1);
+    const SegLds G = seg_lds(lds_dyn, Bp1);
+    seg_clear(G);
+    uint32_t *mt_lds = (uint32_t *)(lds_dyn + seg_layout(Bp1).total);
     const bool need_mt = any_binding(R);
     WaveRng rng;
     rng_load(R, rng, mt_lds, need_mt);
@@ -51,10 +51,10 @@ __global__ void __launch_bounds__(64) k_mc_run(const chromo_replica *reps, int n
     __syncthreads();
     int err = 0;
     switch (R->B) {
-    case 1: run_steps<1>(R, rng, K, sctrl, nc, steps, err); break;
-    case 2: run_steps<2>(R, rng, K, sctrl, nc, steps, err); break;
-    case 3: run_steps<3>(R, rng, K, sctrl, nc, steps, err); break;
-    case 4: run_steps<4>(R, rng, K, sctrl, nc, steps, err); break;
+    case 1: run_steps<1>(R, rng, K, sctrl, nc, steps, err, G); break;
+    case 2: run_steps<2>(R, rng, K, sctrl, nc, steps, err, G); break;
+    case 3: run_steps<3>(R, rng, K, sctrl, nc, steps, err, G); break;
+    case 4: run_steps<4>(R, rng, K, sctrl, nc, steps, err, G); break;
     default: err = ERR_NBINDERS; break;
     }
     rng_store(R, rng, need_mt);
@@ -74,9 +74,9 @@ __global__ void __launch_bounds__(64) k_mc_step(const chromo_replica *reps, int 
         return;
     }
     const int Bp1 = R->B + 1;
-    ArenaLayout L = lds_layout(Bp1);
-    uint32_t *mt_lds = (uint32_t *)(lds_dyn + L.total);
-    lds_hash_clear(Bp1);
+    const SegLds G = seg_lds(lds_dyn, Bp1);
+    seg_clear(G);
+    uint32_t *mt_lds = (uint32_t *)(lds_dyn + seg_layout(Bp1).total);
     chromo_ctrl C = R->ctrl[ci];
     const bool need_mt = (C.move == CHROMO_CHANGE_BINDING_STATE);
     WaveRng rng;
@@ -85,10 +85,10 @@ __global__ void __launch_bounds__(64) k_mc_step(const chromo_replica *reps, int 
     counters_load(R, K);
     MoveOut o;
     switch (R->B) {
-    case 1: mc_step_dev<1>(R, rng, C, o); break;
-    case 2: mc_step_dev<2>(R, rng, C, o); break;
-    case 3: mc_step_dev<3>(R, rng, C, o); break;
-    case 4: mc_step_dev<4>(R, rng, C, o); break;
+    case 1: mc_step_dev<1>(R, rng, C, o, G); break;
+    case 2: mc_step_dev<2>(R, rng, C, o, G); break;
+    case 3: mc_step_dev<3>(R, rng, C, o, G); break;
+    case 4: mc_step_dev<4>(R, rng, C, o, G); break;
     default: o.err = ERR_NBINDERS; o.n = 0; break;
     }
     if (!o.err && o.n > 0) record(R, C, o, K);
@@ -440,11 +440,12 @@ int check_reps(const chromo_replica *d_reps, int32_t n_reps)
     return 0;
 }
 
-size_t g_lds_bytes = 0;
-int g_lds_bp1 = 0;
+size_t g_lds_bytes = 0;   // field tape: LDS move arena + MT block
+int g_lds_bp1 = 0;        // largest binder count + 1 the tape's arena fits
+size_t g_mc_bytes = 0;    // Monte Carlo kernels: evaluator block + MT block, for B = 4
 
-// Dynamic LDS per workgroup: the move arena plus the MT19937 block, sized for
-// the largest binder count that fits the device's per-workgroup LDS limit.
+// Dynamic LDS per workgroup, sized for the largest binder count that fits the
+// device's per-workgroup limit, set once per process.
 int ensure_lds()
 {
     if (g_lds_bytes) return 0;
@@ -454,14 +455,15 @@ int ensure_lds()
     int bp1 = CHROMO_MAX_BINDERS + 1;
     while (bp1 > 1 && lds_bytes_for(bp1) + 1024 > (size_t)lim) --bp1;
     if (bp1 < 2) return set_err(-2, "device LDS per workgroup too small for the move arena");
+    const size_t mc = mc_lds_bytes(CHROMO_MAX_BINDERS + 1);
+    if (mc + 1024 > (size_t)lim) return set_err(-2, "device LDS per workgroup too small for the evaluator");
+    HIPC(hipFuncSetAttribute((const void *)k_mc_run, hipFuncAttributeMaxDynamicSharedMemorySize, (int)mc));
+    HIPC(hipFuncSetAttribute((const void *)k_mc_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)mc));
+    HIPC(hipFuncSetAttribute((const void *)k_field_tape, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds_bytes_for(bp1)));
+    g_mc_bytes = mc;
     g_lds_bp1 = bp1;
     g_lds_bytes = lds_bytes_for(bp1);
-    HIPC(hipFuncSetAttribute((const void *)k_mc_run, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)g_lds_bytes));
-    HIPC(hipFuncSetAttribute((const void *)k_mc_step, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)g_lds_bytes));
-    HIPC(hipFuncSetAttribute((const void *)k_field_tape,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)g_lds_bytes));
     return 0;
 }
 
@@ -583,8 +585,8 @@ int chromo_mc_run(const chromo_replica *d_reps, int32_t n_reps, int64_t num_step
     if (num_steps < 0) return set_err(-1, "num_steps < 0");
     rc = ensure_lds();
     if (rc) return rc;
-    hipLaunchKernelGGL(k_mc_run, dim3(n_reps), dim3(64), g_lds_bytes, (hipStream_t)stream, d_reps,
-                       n_reps, num_steps, g_lds_bp1);
+    hipLaunchKernelGGL(k_mc_run, dim3(n_reps), dim3(64), g_mc_bytes, (hipStream_t)stream, d_reps,
+                       n_reps, num_steps, CHROMO_MAX_BINDERS + 1);
     HIPC(launch_err());
     return 0;
 }
@@ -608,14 +610,13 @@ int chromo_mc_run_tuned(const chromo_replica *d_reps, int32_t n_reps, int64_t nu
     int bp1 = 0;
     for (int b = 1; b <= CHROMO_MAX_BINDERS; ++b)
         if (binder_mask & (1 << b)) bp1 = b + 1;
-    if (bp1 > g_lds_bp1) return set_err(-1, "binder count exceeds what the device LDS holds");
     hipStream_t st = (hipStream_t)stream;
     // replicas whose binder count is not in binder_mask fail with ERR_NBINDERS
     hipLaunchKernelGGL(k_check_binders, dim3((n_reps + 63) / 64), dim3(64), 0, st, d_reps, n_reps,
                        binder_mask);
     HIPC(launch_err());
     if (W == 1) {
-        hipLaunchKernelGGL(k_mc_run, dim3(n_reps), dim3(WAVE), lds_bytes_for(bp1), st, d_reps, n_reps,
+        hipLaunchKernelGGL(k_mc_run, dim3(n_reps), dim3(WAVE), mc_lds_bytes(bp1), st, d_reps, n_reps,
                            num_steps, bp1);
         HIPC(launch_err());
         return 0;
@@ -632,8 +633,8 @@ int chromo_mc_step(const chromo_replica *d_reps, int32_t rep, int32_t ctrl, void
     if (d_reps == nullptr || rep < 0) return set_err(-1, "bad replica");
     int rc = ensure_lds();
     if (rc) return rc;
-    hipLaunchKernelGGL(k_mc_step, dim3(1), dim3(64), g_lds_bytes, (hipStream_t)stream, d_reps, rep,
-                       ctrl, g_lds_bp1);
+    hipLaunchKernelGGL(k_mc_step, dim3(1), dim3(64), g_mc_bytes, (hipStream_t)stream, d_reps, rep,
+                       ctrl, CHROMO_MAX_BINDERS + 1);
     HIPC(launch_err());
     return 0;
 }
@@ -664,8 +665,8 @@ int chromo_mc_run_plain(const chromo_replica *d_reps, int32_t rep, int64_t num_s
         for (int ci = 0; ci < h.n_ctrl; ++ci) {
             if (c[ci].on == 1)
                 for (int j = 0; j < c[ci].num_per_cycle; ++j)
-                    hipLaunchKernelGGL(k_mc_step, dim3(1), dim3(64), g_lds_bytes, st, d_reps, rep, ci,
-                                       g_lds_bp1);
+                    hipLaunchKernelGGL(k_mc_step, dim3(1), dim3(64), g_mc_bytes, st, d_reps, rep, ci,
+                                       CHROMO_MAX_BINDERS + 1);
             hipLaunchKernelGGL(k_update_amplitudes, dim3(1), dim3(64), 0, st, d_reps, rep, ci);
         }
         HIPC(launch_err());

@@ -1,13 +1,14 @@
 // chromo_mc.hpp -- device code of one Monte Carlo move, shared by the kernels.
 //
 // One wavefront runs a replica's sequential Markov chain
-// (chromo/mc/mc_sim.pyx:26-182).  A move's affected beads are spread over the
-// 64 lanes; its 16*n voxel contributions (2 configurations x 8 corners per bead)
-// are grouped per voxel in an arena (LDS for n <= SEG_LDS, per-replica global
-// scratch above that) through a hash table, ordered by contribution index and
-// folded per voxel in the reference's order, so each voxel's delta density is
-// the reference's left fold (chromo/fields.pyx:1491-1520) bit for bit.  Energy
-// terms are reduced with wavefront shuffles; no global atomics are used.
+// (chromo/mc/mc_sim.pyx:26-182).  Moves of at most SEG_MAX beads are evaluated
+// by seg_eval (chromo_seg.hpp).  Larger ones spread their 16*n voxel
+// contributions (2 configurations x 8 corners per bead) over the 64 lanes and
+// group them per voxel in the replica's global scratch arena through a hash
+// table, ordered by contribution index and folded per voxel in the reference's
+// order, so each voxel's delta density is the reference's left fold
+// (chromo/fields.pyx:1491-1520) bit for bit.  Energy terms are reduced across the
+// wavefront in a fixed order; no global atomics are used.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,6 +21,23 @@
 namespace {
 
 using namespace chromo;
+
+struct Proposal {
+    int move;
+    int64_t n, ind0, indf;
+    double T[12];          // crank / end pivot rotation, slide translation in T[3,7,11]
+    int binder;
+    // per-lane registers for beads i == lane (< 64)
+    int64_t mypick;        // tangent bead index
+    double myT[9];         // tangent rotation
+    int mynew;             // binding new state
+};
+
+}  // namespace
+
+#include "chromo_seg.hpp"
+
+namespace {
 
 // ---------------------------------------------------------------------------
 // Move arena
@@ -130,11 +148,16 @@ __device__ inline Arena make_arena(char *base, const ArenaLayout &L)
 
 constexpr size_t MT_LDS_BYTES = 624 * sizeof(uint32_t);
 
+// Dynamic LDS of the field-dE tape kernel: the LDS move arena and an MT block.
 inline size_t lds_bytes_for(int Bp1)
 {
     ArenaLayout L = lds_layout(Bp1);
     return L.total + MT_LDS_BYTES;
 }
+
+// Dynamic LDS of the one-wavefront Monte Carlo kernels: one evaluator block, then
+// the MT19937 state.
+inline size_t mc_lds_bytes(int Bp1) { return seg_layout(Bp1).total + MT_LDS_BYTES; }
 
 extern __shared__ char lds_dyn[];
 
@@ -154,17 +177,6 @@ struct MoveOut {
 // KIND 1 = global scratch arena.
 // ---------------------------------------------------------------------------
 
-struct Proposal {
-    int move;
-    int64_t n, ind0, indf;
-    double T[12];          // crank / end pivot rotation, slide translation in T[3,7,11]
-    int binder;
-    // per-lane registers for beads i == lane (< 64)
-    int64_t mypick;        // tangent bead index
-    double myT[9];         // tangent rotation
-    int mynew;             // binding new state
-};
-
 // The arena is LDS (KIND 0) or global scratch (KIND 1); one wavefront uses it.
 template <int KIND>
 __device__ __forceinline__ void arena_sync()
@@ -174,7 +186,7 @@ __device__ __forceinline__ void arena_sync()
 }
 
 template <int KIND, int NB>
-__device__ __forceinline__ void move_body(const chromo_replica *R, WaveRng &rng, Proposal &P, MoveOut &out)
+__device__ __forceinline__ void move_body(const chromo_replica *R, double u, Proposal &P, MoveOut &out)
 {
     const int lane = lane_id();
     const int64_t N = R->N;
@@ -529,7 +541,6 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, WaveRng &rng,
     dE += dEp;
     if (check_field) dE += dEf;
     double expdE = exp(-dE);
-    double u = rnd_unit(rng);
     int acc = u < expdE;
 
     // ---- commit (MCAdapter.accept, moves.pyx:190-226; update_affected_densities,
@@ -586,7 +597,8 @@ __device__ __forceinline__ void load_pos3(const double *Pp, int64_t N, int64_t b
 }
 
 template <int NB>
-__device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rng, chromo_ctrl &C, MoveOut &out)
+__device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rng, chromo_ctrl &C, MoveOut &out,
+                                            const SegLds &G)
 {
     const int lane = lane_id();
     const int64_t N = R->N;
@@ -646,7 +658,7 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
     } else if (P.move == CHROMO_END_PIVOT) {
         // end_pivot, move_funcs.pyx:285-346
         double ang = amp_move * (((double)rnd31(rng) / RMAX) - 0.5);
-        int64_t lhs = rnd31(rng) % 2;
+        int64_t lhs = rnd31(rng) & 1;
         if (lhs == 1) {
             ind0 = 0;
             indf = from_left(rng, amp_bead, N, err) + 1;
@@ -678,7 +690,7 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
         P.T[3] = dir[0];
         P.T[7] = dir[1];
         P.T[11] = dir[2];
-        int64_t b0 = rnd31(rng) % N;
+        int64_t b0 = draw_mod(rnd31(rng), N);
         int64_t b1 = from_point(rng, amp_bead, N, b0, err);
         if (err) goto fail;
         check_bead_bounds(b0, b1, N, ind0, indf);
@@ -686,26 +698,25 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
     } else if (P.move == CHROMO_TANGENT_ROTATION) {
         // tangent_rotation + get_inds + rotate_select_beads, move_funcs.pyx:470-582
         double ang = amp_move * (((double)rnd31(rng) / RMAX) - 0.5);
-        int64_t k = rnd31(rng) % amp_bead + 1;
+        int64_t k = draw_mod(rnd31(rng), amp_bead) + 1;
         if (k > N) {
             err = ERR_TANGENT_K;
             goto fail;
         }
-        if (k > SEG_LDS && k > R->bead_cap) {
+        if (k > SEG_MAX && k > R->bead_cap) {
             err = ERR_SEGCAP;
             goto fail;
         }
         n = k;
-        Arena A;
-        if (k <= SEG_LDS) A = make_arena(lds_dyn, lds_layout(NB + 1));
-        else A = make_arena((char *)R->scratch, global_layout(R->bead_cap, R->seg_cap, NB + 1));
+        // beads past the first 64 keep their picks and rotations in the global arena
+        Arena A = make_arena((char *)R->scratch, global_layout(R->bead_cap, R->seg_cap, NB + 1));
         for (int64_t i = 0; i < k && !err; ++i) {
             for (int it = 0;; ++it) {
                 if (it >= SPIN_LIMIT) {
                     err = ERR_SPIN;
                     break;
                 }
-                int64_t c = rnd31(rng) % N;
+                int64_t c = draw_mod(rnd31(rng), N);
                 int dup = (lane < i && P.mypick == c);
                 for (int64_t j = WAVE + lane; j < i; j += WAVE) dup |= (A.inds[j] == c);
                 if (!__any(dup)) {
@@ -746,19 +757,17 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
         }
         int64_t binder = rnd31(rng) % NB;
         int64_t ntails = R->sites[binder];
-        int64_t b0 = rnd31(rng) % N;
+        int64_t b0 = draw_mod(rnd31(rng), N);
         int64_t b1 = from_point(rng, amp_bead, N, b0, err);
         if (err) goto fail;
         check_bead_bounds(b0, b1, N, ind0, indf);
         n = indf - ind0;
         P.binder = (int)binder;
-        if (n > SEG_LDS && (n > R->seg_cap || n > R->bead_cap)) {
+        if (n > SEG_MAX && (n > R->seg_cap || n > R->bead_cap)) {
             err = ERR_SEGCAP;
             goto fail;
         }
-        Arena A;
-        if (n <= SEG_LDS) A = make_arena(lds_dyn, lds_layout(NB + 1));
-        else A = make_arena((char *)R->scratch, global_layout(R->bead_cap, R->seg_cap, NB + 1));
+        Arena A = make_arena((char *)R->scratch, global_layout(R->bead_cap, R->seg_cap, NB + 1));
         for (int64_t i = 0; i < n; ++i) {
             int st = (int)rnd_randint(rng, ntails + 1);
             if (i < WAVE) {
@@ -781,15 +790,42 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
         out.ind0 = readlane_i64(P.mypick, 0);
         out.indf = out.ind0 + 1;
     }
-    if (n > SEG_LDS) {
+    if (n > SEG_MAX) {
         const bool uses_field = R->field_on && P.move != CHROMO_TANGENT_ROTATION;
         if (n > R->bead_cap || (uses_field && n > R->seg_cap)) {
             err = ERR_SEGCAP;
             goto fail;
         }
     }
-    if (n <= SEG_LDS) move_body<0, NB>(R, rng, P, out);
-    else move_body<1, NB>(R, rng, P, out);
+    {
+        // the Metropolis draw follows the proposal's draws (mc_sim.pyx:171)
+        const double u = rnd_unit(rng);
+        SegRes sr;
+        sr.overflow = 1;
+        if (n <= SEG_MAX) {
+            SegKeep<NB> K;
+            seg_eval<NB>(R, P, u, G, sr, K);
+            if (!sr.overflow) {
+                if (sr.accepted) seg_commit<NB>(R, P.move, n, P.binder, P.mynew, K, G, sr.U);
+                wave_sync_global();
+                out.dEp = sr.dEp;
+                out.dEf = sr.dEf;
+                out.u = u;
+                out.expdE = sr.expdE;
+                out.accepted = sr.accepted;
+                out.nbins = sr.U;
+            }
+        }
+        // larger moves, and the rare small one with too many voxels for seg_eval
+        if (sr.overflow) {
+            const bool uses_field = R->field_on && P.move != CHROMO_TANGENT_ROTATION;
+            if (n > R->bead_cap || (uses_field && n > R->seg_cap)) {
+                err = ERR_SEGCAP;
+                goto fail;
+            }
+            move_body<1, NB>(R, u, P, out);
+        }
+    }
     if (out.err) {
         out.n = 0;
         return;
@@ -851,23 +887,24 @@ __device__ __forceinline__ void rng_load(const chromo_replica *R, WaveRng &rng, 
 {
     const int lane = lane_id();
     const uint32_t *w = R->rng;
-    rng.mode = R->rng_mode;
+    // every field but the glibc words is wave-uniform: scalar registers
+    rng.mode = uni_i32(R->rng_mode);
     rng.g = lane < 31 ? w[CHROMO_RNG_GLIBC_STATE + lane] : 0u;
-    rng.gf = (int)w[CHROMO_RNG_GLIBC_F];
-    rng.gr = (int)w[CHROMO_RNG_GLIBC_R];
+    rng.gf = uni_i32((int)w[CHROMO_RNG_GLIBC_F]);
+    rng.gr = uni_i32((int)w[CHROMO_RNG_GLIBC_R]);
     rng.mt = mt_lds;
-    rng.mti = (int)w[CHROMO_RNG_MT_INDEX];
+    rng.mti = uni_i32((int)w[CHROMO_RNG_MT_INDEX]);
     if (need_mt && rng.mode == CHROMO_RNG_GLIBC) {
         for (int i = lane; i < 624; i += WAVE) mt_lds[i] = w[CHROMO_RNG_MT_STATE + i];
     }
-    rng.k0 = w[CHROMO_RNG_PHILOX_KEY];
-    rng.k1 = w[CHROMO_RNG_PHILOX_KEY + 1];
-    rng.ctr = (uint64_t)w[CHROMO_RNG_PHILOX_CTR] | ((uint64_t)w[CHROMO_RNG_PHILOX_CTR + 1] << 32);
-    rng.b0 = w[CHROMO_RNG_PHILOX_BUF];
-    rng.b1 = w[CHROMO_RNG_PHILOX_BUF + 1];
-    rng.b2 = w[CHROMO_RNG_PHILOX_BUF + 2];
-    rng.b3 = w[CHROMO_RNG_PHILOX_BUF + 3];
-    rng.bi = (int)w[CHROMO_RNG_PHILOX_IDX];
+    rng.k0 = uni_u32(w[CHROMO_RNG_PHILOX_KEY]);
+    rng.k1 = uni_u32(w[CHROMO_RNG_PHILOX_KEY + 1]);
+    rng.ctr = uni_u64((uint64_t)w[CHROMO_RNG_PHILOX_CTR] | ((uint64_t)w[CHROMO_RNG_PHILOX_CTR + 1] << 32));
+    rng.b0 = uni_u32(w[CHROMO_RNG_PHILOX_BUF]);
+    rng.b1 = uni_u32(w[CHROMO_RNG_PHILOX_BUF + 1]);
+    rng.b2 = uni_u32(w[CHROMO_RNG_PHILOX_BUF + 2]);
+    rng.b3 = uni_u32(w[CHROMO_RNG_PHILOX_BUF + 3]);
+    rng.bi = uni_i32((int)w[CHROMO_RNG_PHILOX_IDX]);
     wave_sync();
 }
 
@@ -980,7 +1017,7 @@ __device__ __forceinline__ bool any_binding(const chromo_replica *R)
 
 template <int NB>
 __device__ __forceinline__ void run_steps(const chromo_replica *R, WaveRng &rng, Counters &K, chromo_ctrl *sctrl,
-                          int nc, int64_t steps, int &err)
+                          int nc, int64_t steps, int &err, const SegLds &G)
 {
     for (int64_t s = 0; s < steps && !err; ++s) {
         for (int ci = 0; ci < nc && !err; ++ci) {
@@ -988,7 +1025,7 @@ __device__ __forceinline__ void run_steps(const chromo_replica *R, WaveRng &rng,
             if (c.on == 1) {
                 for (int j = 0; j < c.num_per_cycle; ++j) {
                     MoveOut o;
-                    mc_step_dev<NB>(R, rng, c, o);
+                    mc_step_dev<NB>(R, rng, c, o, G);
                     if (o.err) {
                         err = o.err;
                         break;

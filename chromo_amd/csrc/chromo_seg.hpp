@@ -1,0 +1,498 @@
+// chromo_seg.hpp -- evaluation of one move of at most SEG_MAX beads by one
+// wavefront.
+//
+// Lane i holds bead i (i < n): its current and trial coordinates, states and
+// elastic partials.  A move has 2n voxel positions p = 2i + k (k = 0 current,
+// k = 1 trial; chromo/fields.pyx:1434-1520 walks them in this order) and each
+// position touches the 8 corners of its voxel cell, so contribution e = 8p + l.
+//
+//  1. The 16n contributions, 64 at a time in contribution order, enter a
+//     per-wavefront LDS hash keyed by voxel that keeps each voxel's smallest
+//     contribution index; a wave scan then numbers the first occurrences.  This
+//     is the reference's list of unique voxels in first-occurrence order
+//     (U = n_bins).
+//  2. Voxel u goes to lane u mod 64 (at most SEG_VB per lane).  The lane walks
+//     the positions from the voxel's first one on and folds every contribution
+//     that lands in it, in contribution order, with the 1e-18 rules of
+//     fields.pyx:1491-1520: the per-voxel delta density is the reference's left
+//     fold bit for bit.  The new densities stay in registers until the commit.
+//  3. Energy terms, wave reduction (fixed order), Metropolis.
+//
+// A move with more than SEG_VB * 64 unique voxels is reported as an overflow;
+// the caller evaluates it through the global-arena path instead.
+#pragma once
+
+#include "chromo_device.hpp"
+
+namespace {
+
+using namespace chromo;
+
+constexpr int SEG_MAX = 32;                   // beads per move
+constexpr int SEG_POS = 2 * SEG_MAX;          // positions (<= one per lane)
+constexpr int SEG_E = 8 * SEG_POS;            // contributions
+constexpr int SEG_ROUNDS = SEG_E / 64;        // contribution rounds of 64 lanes
+constexpr int SEG_VB = 4;                     // unique voxels per lane
+constexpr int SEG_UMAX = SEG_VB * 64;         // unique voxels one evaluation holds
+constexpr int SEG_HASH = 1024;                // >= 2 * SEG_E
+constexpr int SEG_HASH_SHIFT = 22;            // 32 - log2(SEG_HASH)
+static_assert(SEG_POS <= 64, "one position per lane");
+
+struct SegLayout {
+    size_t c0, prec, hkey, hmin, eslot, uvox, ucrd, uaux, total;
+};
+
+// Per-wavefront LDS block of the evaluator.
+__host__ __device__ inline SegLayout seg_layout(int)
+{
+    SegLayout S;
+    size_t o = 0;
+    S.c0 = o;    o += sizeof(double) * SEG_E;          // corner weight / vol_bin per contribution
+    S.prec = o;  o += 16 * (size_t)SEG_POS;            // position records (int4)
+    S.hkey = o;  o += sizeof(int32_t) * SEG_HASH;      // voxel -> first contribution
+    S.hmin = o;  o += sizeof(int32_t) * SEG_HASH;
+    S.eslot = o; o += sizeof(int32_t) * SEG_E;         // hash slot per contribution
+    S.uvox = o;  o += sizeof(int32_t) * SEG_UMAX;      // unique voxel index
+    S.ucrd = o;  o += 8 * (size_t)SEG_UMAX;            // its cell (int2: x | y << 16, z | first position << 16)
+    S.uaux = o;  o += sizeof(int32_t) * SEG_UMAX;      // hash slot; later a conflict-table slot
+    S.total = (o + 255) / 256 * 256;
+    return S;
+}
+
+struct SegLds {
+    double *c0;
+    int4 *prec;
+    int32_t *hkey, *hmin, *eslot, *uvox, *uaux;
+    int2 *ucrd;
+};
+
+__device__ __forceinline__ SegLds seg_lds(char *base, int Bp1)
+{
+    const SegLayout L = seg_layout(Bp1);
+    SegLds S;
+    S.c0 = (double *)(base + L.c0);
+    S.prec = (int4 *)(base + L.prec);
+    S.hkey = (int32_t *)(base + L.hkey);
+    S.hmin = (int32_t *)(base + L.hmin);
+    S.eslot = (int32_t *)(base + L.eslot);
+    S.uvox = (int32_t *)(base + L.uvox);
+    S.ucrd = (int2 *)(base + L.ucrd);
+    S.uaux = (int32_t *)(base + L.uaux);
+    return S;
+}
+
+// Empties the evaluator's hash (once per launch, by its wavefront).
+__device__ __forceinline__ void seg_clear(const SegLds &S)
+{
+    for (int i = lane_id(); i < SEG_HASH; i += WAVE) {
+        S.hkey[i] = HKEY_EMPTY;
+        S.hmin[i] = HMIN_EMPTY;
+    }
+    wave_sync();
+}
+
+// Wave-uniform results of one evaluation.
+struct SegRes {
+    double dEp, dEf, expdE;
+    int U, accepted, overflow;
+};
+
+// Per-lane state kept from the evaluation to the commit: the trial coordinates of
+// bead i == lane and the new densities of voxels lane + 64 j.
+template <int NB>
+struct SegKeep {
+    double w[9];
+    int64_t b;
+    double nr[SEG_VB][NB + 1];
+};
+
+__device__ __forceinline__ int64_t pack_ix(const int64_t ix[3])
+{
+    return ix[0] | (ix[1] << 21) | (ix[2] << 42);
+}
+
+__device__ __forceinline__ bool axis_hit(int c, int d, int64_t nn)
+{
+    return nn == 1 ? true : (c == 0 ? d == 0 : d == 1);
+}
+
+// One contribution of the per-voxel fold (fields.pyx:1491-1520); c0 is the corner
+// weight divided by vol_bin, sq the packed binder states of the position.
+template <int NB>
+__device__ __forceinline__ void fold_c0(double c0, double pre, int32_t sq, double *acc)
+{
+    const double t0 = pre * c0;
+    if (fabs(t0) > 1E-18) acc[0] += t0;
+#pragma unroll
+    for (int m = 1; m <= NB; ++m) {
+        const int st = (sq >> (8 * (m - 1))) & 0xff;
+        const double t = pre * (c0 * (double)st);
+        if (fabs(t) > 1E-18) acc[m] += t;
+    }
+}
+
+// Trial coordinates of bead (o) under proposal P, lane-local (move_funcs.pyx:139-154).
+__device__ __forceinline__ void seg_transform(const Proposal &P, const double o[9], double w[9])
+{
+    const int mv = P.move;
+    if (mv == CHROMO_CRANK_SHAFT || mv == CHROMO_END_PIVOT) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            w[j] = rowdot(&P.T[4 * j], o[0], o[1], o[2]) + P.T[4 * j + 3];
+            w[3 + j] = rowdot(&P.T[4 * j], o[3], o[4], o[5]);
+            w[6 + j] = rowdot(&P.T[4 * j], o[6], o[7], o[8]);
+        }
+    } else if (mv == CHROMO_SLIDE) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) w[j] = o[j] + P.T[4 * j + 3];
+#pragma unroll
+        for (int c = 3; c < 9; ++c) w[c] = o[c];
+    } else if (mv == CHROMO_TANGENT_ROTATION) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            w[j] = o[j];
+            w[3 + j] = rowdot(&P.myT[3 * j], o[3], o[4], o[5]);
+            w[6 + j] = rowdot(&P.myT[3 * j], o[6], o[7], o[8]);
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) w[c] = o[c];
+    }
+}
+
+__device__ __forceinline__ int wrap1(int c, int nn) { return c + 1 == nn ? 0 : c + 1; }
+
+// Evaluates proposal P (1 <= n <= SEG_MAX) against the committed state: polymer
+// dE (SSWLC.compute_dE, polymers.pyx:1024-1082), field dE
+// (UniformDensityField.compute_dE, fields.pyx:1149-1233) and the Metropolis test
+// with the uniform draw u (mc_sim.pyx:152-182).  Writes nothing outside S and K.
+// Called by one whole wavefront; S's hash is empty on entry and on return.
+template <int NB>
+__device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal &P, double u, const SegLds &S,
+                                         SegRes &res, SegKeep<NB> &K)
+{
+    constexpr int Bp1 = NB + 1;
+    const int lane = lane_id();
+    const int64_t N = R->N;
+    const double *Pp = R->pos;
+    const int mv = P.move;
+    const int64_t n = P.n, ind0 = P.ind0, indf = P.indf;
+    const bool tangent = mv == CHROMO_TANGENT_ROTATION;
+    const bool state_change = mv == CHROMO_CHANGE_BINDING_STATE;
+    const bool check_field = R->field_on && !tangent;
+    const bool act = lane < n;
+    res.overflow = 0;
+
+    // ---- bead i == lane: coordinates, states, trial
+    int64_t b = tangent ? P.mypick : ind0 + lane;
+    if (!act) b = tangent ? 0 : ind0;
+    K.b = b;
+    double o[9];
+    int8_t so[NB], sn[NB];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) o[c] = 0;
+#pragma unroll
+    for (int m = 0; m < NB; ++m) so[m] = sn[m] = 0;
+    if (act) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) o[c] = Pp[c * N + b];
+#pragma unroll
+        for (int m = 0; m < NB; ++m) {
+            so[m] = R->states[(int64_t)m * N + b];
+            sn[m] = so[m];
+        }
+        if (state_change) {
+#pragma unroll
+            for (int m = 0; m < NB; ++m)
+                if (m == P.binder) sn[m] = (int8_t)P.mynew;
+        }
+    }
+    double w[9];
+    seg_transform(P, o, w);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) K.w[c] = w[c];
+
+    // ---- polymer dE
+    double dEp = 0;
+    if (tangent || state_change) {
+        double e = 0;
+        if (act && tangent) {
+            if (b != 0) {
+                double r0[3], t30[3];
+                for (int j = 0; j < 3; ++j) {
+                    r0[j] = Pp[j * N + b - 1];
+                    t30[j] = Pp[(3 + j) * N + b - 1];
+                }
+                Bond bd = load_bond(R->bonds, N, b - 1);
+                e += pair_dE_forward(bd, r0, &o[0], &w[0], t30, &o[3], &w[3]);
+            }
+            if (b + 1 != N) {
+                double r1[3], t31[3];
+                for (int j = 0; j < 3; ++j) {
+                    r1[j] = Pp[j * N + b + 1];
+                    t31[j] = Pp[(3 + j) * N + b + 1];
+                }
+                Bond bd = load_bond(R->bonds, N, b);
+                e += pair_dE_reverse(bd, &o[0], &w[0], r1, &o[3], &w[3], t31);
+            }
+        } else if (act) {
+            int8_t mods[NB];
+#pragma unroll
+            for (int m = 0; m < NB; ++m) mods[m] = R->marks[(int64_t)m * N + b];
+            e = bead_binding_dE<NB>(R, mods, so, sn);
+        }
+        for (int i = 0; i < n; ++i) dEp += readlane_f64(e, i);
+    } else {
+        double ef = 0, er = 0;
+        if (lane == 0 && ind0 != 0) {
+            double r0[3], t30[3];
+            for (int j = 0; j < 3; ++j) {
+                r0[j] = Pp[j * N + ind0 - 1];
+                t30[j] = Pp[(3 + j) * N + ind0 - 1];
+            }
+            Bond bd = load_bond(R->bonds, N, ind0 - 1);
+            ef = pair_dE_forward(bd, r0, &o[0], &w[0], t30, &o[3], &w[3]);
+        }
+        if (lane == n - 1 && indf != N) {
+            double r1[3], t31[3];
+            for (int j = 0; j < 3; ++j) {
+                r1[j] = Pp[j * N + indf];
+                t31[j] = Pp[(3 + j) * N + indf];
+            }
+            Bond bd = load_bond(R->bonds, N, indf - 1);
+            er = pair_dE_reverse(bd, &o[0], &w[0], r1, &o[3], &w[3], t31);
+        }
+        if (ind0 != 0) dEp += readlane_f64(ef, 0);
+        if (indf != N) dEp += readlane_f64(er, (int)(n - 1));
+    }
+
+    // ---- field dE
+    double dEf = 0;
+    int U = 0;
+    if (check_field) {
+        const int nx = (int)R->nx, ny = (int)R->ny, nz = (int)R->nz;
+        const double vol_bin = R->vol_bin;
+        // positions 2i (current) and 2i + 1 (trial; the current one for a state change)
+        if (act) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                double pt[3];
+                for (int j = 0; j < 3; ++j) pt[j] = (k == 0 || state_change) ? o[j] : w[j];
+                int64_t ix[3];
+                double wt[3];
+                voxel_lower(R, pt, ix, wt);
+                const int p = 2 * lane + k;
+                const int x = (int)ix[0], y = (int)ix[1], z = (int)ix[2];
+                int32_t sp = 0;
+#pragma unroll
+                for (int m = 0; m < NB; ++m)
+                    sp |= ((int32_t)(uint8_t)((k == 0 || !state_change) ? so[m] : sn[m])) << (8 * m);
+                S.prec[p] = make_int4(x | (wrap1(x, nx) << 16), y | (wrap1(y, ny) << 16),
+                                      z | (wrap1(z, nz) << 16), sp);
+#pragma unroll
+                for (int l = 0; l < 8; ++l) S.c0[p * 8 + l] = corner_weight(l, wt[0], wt[1], wt[2]) / vol_bin;
+            }
+        }
+        wave_sync();
+
+        // 1. unique voxels: hash the contributions in order, keep each voxel's first
+        const int E = (int)(16 * n);
+        const int rounds = (E + WAVE - 1) / WAVE;
+#pragma unroll
+        for (int r = 0; r < SEG_ROUNDS; ++r) {
+            if (r >= rounds) break;
+            const int e = lane + WAVE * r;
+            if (e < E) {
+                const int4 pr = S.prec[e >> 3];
+                const int l = e & 7;
+                const int x = (l & 1) ? (pr.x >> 16) : (pr.x & 0xffff);
+                const int y = (l & 2) ? (pr.y >> 16) : (pr.y & 0xffff);
+                const int z = (l & 4) ? (pr.z >> 16) : (pr.z & 0xffff);
+                const int v = x + nx * (y + ny * z);
+                uint32_t h = ((uint32_t)v * 2654435761u) >> SEG_HASH_SHIFT;
+                // at most SEG_E keys in 2 * SEG_E slots: the probe ends
+                for (int probes = 0; probes < SEG_HASH; ++probes) {
+                    const int old = atomicCAS(&S.hkey[h], HKEY_EMPTY, v);
+                    if (old == HKEY_EMPTY || old == v) break;
+                    h = (h + 1) & (SEG_HASH - 1);
+                }
+                atomicMin(&S.hmin[h], e);
+                S.eslot[e] = (int)h;
+            }
+        }
+        wave_sync();
+        int total = 0;
+#pragma unroll
+        for (int r = 0; r < SEG_ROUNDS; ++r) {
+            if (r >= rounds) break;
+            const int e = lane + WAVE * r;
+            const int h = e < E ? S.eslot[e] : 0;
+            const bool first = e < E && S.hmin[h] == e;
+            int cnt;
+            const int q = total + wave_excl_scan(first ? 1 : 0, cnt);
+            if (first && q < SEG_UMAX) {
+                const int4 pr = S.prec[e >> 3];
+                const int l = e & 7;
+                const int x = (l & 1) ? (pr.x >> 16) : (pr.x & 0xffff);
+                const int y = (l & 2) ? (pr.y >> 16) : (pr.y & 0xffff);
+                const int z = (l & 4) ? (pr.z >> 16) : (pr.z & 0xffff);
+                S.uvox[q] = x + nx * (y + ny * z);
+                S.ucrd[q] = make_int2(x | (y << 16), z | ((e >> 3) << 16));
+                S.uaux[q] = h;
+            }
+            total += cnt;
+        }
+        U = total;
+        wave_sync();
+        // release the hash (every key is some voxel's first occurrence)
+        for (int q = lane; q < U && q < SEG_UMAX; q += WAVE) {
+            const int h = S.uaux[q];
+            S.hkey[h] = HKEY_EMPTY;
+            S.hmin[h] = HMIN_EMPTY;
+        }
+        if (U > SEG_UMAX) {
+            // too many voxels for the registers: the global-arena path evaluates it
+            for (int e = lane; e < E; e += WAVE) {
+                const int h = S.eslot[e];
+                S.hkey[h] = HKEY_EMPTY;
+                S.hmin[h] = HMIN_EMPTY;
+            }
+            wave_sync();
+            res.overflow = 1;
+            res.U = U;
+            return;
+        }
+        wave_sync();
+
+        // 2. fold per voxel, voxel lane + 64 j on this lane
+        const bool degen = nx == 1 || ny == 1 || nz == 1;
+        const double *rho_g = R->density;
+        const int npos = (int)(2 * n);
+        double S2[NB], SX[NB * NB], NS = 0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) S2[j] = 0;
+#pragma unroll
+        for (int j = 0; j < NB * NB; ++j) SX[j] = 0;
+#pragma unroll
+        for (int j = 0; j < SEG_VB; ++j) {
+#pragma unroll
+            for (int m = 0; m < Bp1; ++m) K.nr[j][m] = 0;
+            if (WAVE * j >= U) continue;
+            const int q = lane + WAVE * j;
+            const bool own = q < U;
+            double rho[Bp1], acc[Bp1];
+#pragma unroll
+            for (int m = 0; m < Bp1; ++m) {
+                rho[m] = 0;
+                acc[m] = 0;
+            }
+            int vx = -1, vy = -1, vz = -1;
+            if (own) {
+                const int v = S.uvox[q];
+#pragma unroll
+                for (int m = 0; m < Bp1; ++m) rho[m] = rho_g[(int64_t)v * Bp1 + m];
+                const int2 c = S.ucrd[q];
+                vx = c.x & 0xffff;
+                vy = c.x >> 16;
+                vz = c.y & 0xffff;
+            }
+            // this batch's voxels first occur at or after its first voxel's position
+            const int p0 = S.ucrd[WAVE * j].y >> 16;
+            for (int p = p0; p < npos; ++p) {
+                const int4 pr = S.prec[p];
+                const double pre = (p & 1) ? 1. : -1.;
+                const int x0 = pr.x & 0xffff, x1 = pr.x >> 16, y0 = pr.y & 0xffff, y1 = pr.y >> 16,
+                          z0 = pr.z & 0xffff, z1 = pr.z >> 16;
+                if (!degen) {
+                    const bool hx = vx == x0 || vx == x1, hy = vy == y0 || vy == y1, hz = vz == z0 || vz == z1;
+                    if (hx && hy && hz) {
+                        const int l = (vx == x1 ? 1 : 0) | (vy == y1 ? 2 : 0) | (vz == z1 ? 4 : 0);
+                        fold_c0<NB>(S.c0[p * 8 + l], pre, pr.w, acc);
+                    }
+                } else {
+                    int dx = vx - x0, dy = vy - y0, dz = vz - z0;
+                    if (dx < 0) dx += nx;
+                    if (dy < 0) dy += ny;
+                    if (dz < 0) dz += nz;
+                    if (own)
+                        for (int l = 0; l < 8; ++l)
+                            if (axis_hit(l & 1, dx, nx) && axis_hit((l >> 1) & 1, dy, ny) &&
+                                axis_hit((l >> 2) & 1, dz, nz))
+                                fold_c0<NB>(S.c0[p * 8 + l], pre, pr.w, acc);
+                }
+            }
+            if (own) {
+                voxel_dE_terms<NB>(R, rho, acc, S2, SX, NS);
+#pragma unroll
+                for (int m = 0; m < Bp1; ++m) K.nr[j][m] = rho[m] + acc[m];
+            }
+        }
+        const double share = wave_sum(lane_field_share<NB>(R, S2, SX, NS));
+
+        // count_doubly_bound (fields.pyx:1877-1937) and confinement (:121-202)
+        int dbl_t[NB], dbl[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            dbl_t[j] = __popcll(__ballot(act && (state_change ? sn[j] : so[j]) == 2));
+            dbl[j] = __popcll(__ballot(act && so[j] == 2));
+        }
+        int out_t = 0, out_c = 0;
+        const int ct = R->confine_type;
+        if (ct != 0 && !state_change) {
+            const double cl = R->confine_length;
+            if (ct == 1) {
+                out_t = __popcll(__ballot(act && sqrt(dot3(&w[0], &w[0])) > cl));
+                out_c = __popcll(__ballot(act && sqrt(dot3(&o[0], &o[0])) > cl));
+            } else {
+#pragma unroll
+                for (int j = 0; j < 3; ++j) out_t += __popcll(__ballot(act && fabs(w[j]) > cl / 2));
+            }
+        }
+        dEf = field_dE_total<NB>(R, share, dbl_t, dbl, out_t, out_c, state_change);
+    }
+
+    // ---- Metropolis
+    double dE = 0;
+    dE += dEp;
+    if (check_field) dE += dEf;
+    res.dEp = dEp;
+    res.dEf = dEf;
+    res.expdE = exp(-dE);
+    res.accepted = u < res.expdE;
+    res.U = U;
+}
+
+// Writes an accepted proposal: bead coordinates or states (MCAdapter.accept,
+// moves.pyx:190-226) and the touched voxels' densities
+// (update_affected_densities, fields.pyx:1968-1975).
+template <int NB>
+__device__ __forceinline__ void seg_commit(const chromo_replica *R, int move, int64_t n, int binder, int mynew,
+                                           const SegKeep<NB> &K, const SegLds &S, int U)
+{
+    constexpr int Bp1 = NB + 1;
+    const int lane = lane_id();
+    const int64_t N = R->N;
+    if (lane < n) {
+        if (move == CHROMO_CHANGE_BINDING_STATE) {
+            R->states[(int64_t)binder * N + K.b] = (int8_t)mynew;
+        } else {
+            const int c0 = (move == CHROMO_TANGENT_ROTATION) ? 3 : 0;
+            const int c1 = (move == CHROMO_SLIDE) ? 3 : 9;
+#pragma unroll
+            for (int c = 0; c < 9; ++c)
+                if (c >= c0 && c < c1) R->pos[c * N + K.b] = K.w[c];
+        }
+    }
+    double *rho = R->density;
+#pragma unroll
+    for (int j = 0; j < SEG_VB; ++j) {
+        const int q = lane + WAVE * j;
+        if (q < U) {
+            const int64_t v = S.uvox[q];
+#pragma unroll
+            for (int m = 0; m < Bp1; ++m) rho[v * Bp1 + m] = K.nr[j][m];
+        }
+    }
+}
+
+}  // namespace

@@ -13,21 +13,21 @@ namespace {
 // accept/reject outcomes before it.  (The exception, a crank-shaft axis shorter
 // than 1e-5 which draws a random axis, is detected and redone one at a time.)
 // Wavefront 0 generates up to W proposals; wavefront w evaluates proposal w
-// against the current state, in registers.  Resolution walks the proposals in
-// order: proposal s keeps its evaluated dE unless an accepted proposal s' < s of
-// the same round wrote a bead or a voxel that s reads; then s and all later
-// proposals go to the next round and are evaluated again.  Each chain therefore
-// is the reference's sequential chain; only the number of rounds depends on
-// conflicts.  Moves of more than SPEC_NMAX beads run one at a time through
-// mc_step_dev.
+// against the current state with seg_eval, in its own LDS block.  Resolution
+// walks the proposals in order: proposal s keeps its evaluated dE unless an
+// accepted proposal s' < s of the same round wrote a bead or a voxel that s
+// reads; then s and all later proposals go to the next round and are evaluated
+// again.  Each chain therefore is the reference's sequential chain; only the
+// number of rounds depends on conflicts.  Moves of more than SPEC_NMAX beads run
+// one at a time through mc_step_dev.
 // ===========================================================================
 
-constexpr int SPEC_NMAX = 8;          // beads (tangent picks) one wavefront evaluates
-constexpr int SPEC_WMAX = 8;          // wavefronts per replica
+constexpr int SPEC_NMAX = SEG_MAX;    // beads (tangent picks) one wavefront evaluates
+constexpr int SPEC_WMAX = 2;          // wavefronts per replica
 constexpr int SPEC_HASH = 1024;       // conflict-table entries
 constexpr int SPEC_HASH_SHIFT = 22;   // 32 - log2(SPEC_HASH)
-constexpr int SPEC_HASH_SOFT = 640;   // insertions by wavefronts 1.. (wavefront 0 adds <= 128)
-
+constexpr int SPEC_HASH_SOFT = 512;   // insertions by wavefronts 1.. (wavefront 0 adds <= SEG_UMAX)
+static_assert(SPEC_HASH_SOFT + SEG_UMAX <= SPEC_HASH, "the conflict table must never fill");
 enum : int { SLOT_SMALL = 0, SLOT_EMPTY = 1, SLOT_SEQ = 2 };
 constexpr int CK_WORDS = 40;
 
@@ -65,17 +65,19 @@ struct SeqIO {
 };
 
 struct SpecLayout {
-    size_t mt, slots, res, hkey, hmask, ctl, seq, total;
+    size_t seg, seg_bytes, mt, slots, res, hkey, hmask, ctl, seq, total;
 };
 
-// LDS: the move arena of the one-at-a-time path, the MT19937 block, two slot
-// buffers, per-wavefront results, the conflict table and the control words.
+// LDS: one evaluator block per wavefront, the MT19937 block, two slot buffers,
+// per-wavefront results, the conflict table and the control words.
 __host__ __device__ inline SpecLayout spec_layout(int Bp1, int W)
 {
     SpecLayout S;
-    ArenaLayout L = lds_layout(Bp1);
-    S.mt = L.total;
-    size_t o = align_up(L.total + MT_LDS_BYTES, 16);
+    S.seg = 0;
+    S.seg_bytes = seg_layout(Bp1).total;
+    size_t o = S.seg_bytes * (size_t)W;
+    S.mt = o;
+    o = align_up(o + MT_LDS_BYTES, 16);
     S.slots = o;
     o += sizeof(SpecSlot) * 2 * (size_t)W;
     S.res = o;
@@ -101,17 +103,9 @@ struct SpecLds {
     SpecCtl *ctl;
     SeqIO *seq;
     uint32_t *mt;
+    SegLds mine;    // this wavefront's evaluator block
+    SegLds first;   // wavefront 0's (the one-at-a-time path)
 };
-
-// Registers a wavefront keeps from its evaluation to its commit.
-struct SpecKeep {
-    double w[9];
-    int64_t b;
-    double nr[2][CHROMO_MAX_BINDERS + 1];
-    int32_t vox[2], hs[2];
-    int own[2], ins[2];
-};
-
 __device__ __forceinline__ void rng_save(const WaveRng &g, uint32_t *ck)
 {
     const int lane = lane_id();
@@ -137,14 +131,14 @@ __device__ __forceinline__ void rng_restore(WaveRng &g, const uint32_t *ck)
     wave_sync();
     const int lane = lane_id();
     g.g = lane < 31 ? ck[lane] : 0u;
-    g.gf = (int)ck[31];
-    g.gr = (int)ck[32];
-    g.ctr = (uint64_t)ck[33] | ((uint64_t)ck[34] << 32);
-    g.b0 = ck[35];
-    g.b1 = ck[36];
-    g.b2 = ck[37];
-    g.b3 = ck[38];
-    g.bi = (int)ck[39];
+    g.gf = uni_i32((int)ck[31]);
+    g.gr = uni_i32((int)ck[32]);
+    g.ctr = uni_u64((uint64_t)ck[33] | ((uint64_t)ck[34] << 32));
+    g.b0 = uni_u32(ck[35]);
+    g.b1 = uni_u32(ck[36]);
+    g.b2 = uni_u32(ck[37]);
+    g.b3 = uni_u32(ck[38]);
+    g.bi = uni_i32((int)ck[39]);
 }
 
 // Draws one proposal into slot S (wavefront 0), in the draw order of
@@ -182,7 +176,7 @@ __device__ __forceinline__ int gen_slot(const chromo_replica *R, WaveRng &rng, c
         seq = err || n > SPEC_NMAX;
     } else if (mv == CHROMO_END_PIVOT) {
         ang = amp_move * (((double)rnd31(rng) / RMAX) - 0.5);
-        lhs = (int32_t)(rnd31(rng) % 2);
+        lhs = (int32_t)(rnd31(rng) & 1);
         if (lhs == 1) {
             ind0 = 0;
             indf = from_left(rng, amp_bead, N, err) + 1;
@@ -201,7 +195,7 @@ __device__ __forceinline__ int gen_slot(const chromo_replica *R, WaveRng &rng, c
         ang = amp_move * ((double)rnd31(rng) / RMAX);
         s1 = rnd31(rng);
         s2 = rnd31(rng);
-        int64_t b0 = rnd31(rng) % N;
+        int64_t b0 = draw_mod(rnd31(rng), N);
         int64_t b1 = from_point(rng, amp_bead, N, b0, err);
         if (!err) {
             check_bead_bounds(b0, b1, N, ind0, indf);
@@ -210,7 +204,7 @@ __device__ __forceinline__ int gen_slot(const chromo_replica *R, WaveRng &rng, c
         seq = err || n > SPEC_NMAX;
     } else if (mv == CHROMO_TANGENT_ROTATION) {
         ang = amp_move * (((double)rnd31(rng) / RMAX) - 0.5);
-        int64_t k = rnd31(rng) % amp_bead + 1;
+        int64_t k = draw_mod(rnd31(rng), amp_bead) + 1;
         n = k;
         seq = (k > N) || (k > SPEC_NMAX);
         if (!seq) {
@@ -222,7 +216,7 @@ __device__ __forceinline__ int gen_slot(const chromo_replica *R, WaveRng &rng, c
                         seq = true;
                         break;
                     }
-                    int64_t c = rnd31(rng) % N;
+                    int64_t c = draw_mod(rnd31(rng), N);
                     if (!__any(lane < i && mine == c)) {
                         if (lane == i) mine = c;
                         break;
@@ -242,7 +236,7 @@ __device__ __forceinline__ int gen_slot(const chromo_replica *R, WaveRng &rng, c
     } else {
         binder = (int32_t)(rnd31(rng) % NB);
         const int64_t ntails = R->sites[binder];
-        int64_t b0 = rnd31(rng) % N;
+        int64_t b0 = draw_mod(rnd31(rng), N);
         int64_t b1 = from_point(rng, amp_bead, N, b0, err);
         if (!err) {
             check_bead_bounds(b0, b1, N, ind0, indf);
@@ -289,18 +283,18 @@ __device__ __forceinline__ int gen_slot(const chromo_replica *R, WaveRng &rng, c
 // callee save most of the register file on every call and the caller spill
 // around it.
 template <int NB, int W>
-__device__ __forceinline__ void seq_step(const chromo_replica *R, SeqIO *io, uint32_t *mt)
+__device__ __forceinline__ void seq_step(const chromo_replica *R, SeqIO *io, uint32_t *mt, const SegLds &G)
 {
     WaveRng rng;
-    rng.mode = R->rng_mode;
+    rng.mode = uni_i32(R->rng_mode);
     rng_restore(rng, io->ck);
     rng.mt = mt;
-    rng.mti = io->mti;
-    rng.k0 = R->rng[CHROMO_RNG_PHILOX_KEY];
-    rng.k1 = R->rng[CHROMO_RNG_PHILOX_KEY + 1];
+    rng.mti = uni_i32(io->mti);
+    rng.k0 = uni_u32(R->rng[CHROMO_RNG_PHILOX_KEY]);
+    rng.k1 = uni_u32(R->rng[CHROMO_RNG_PHILOX_KEY + 1]);
     chromo_ctrl C = io->C;
     MoveOut o;
-    mc_step_dev<NB>(R, rng, C, o);
+    mc_step_dev<NB>(R, rng, C, o, G);
     rng_save(rng, io->ck);
     if (lane_id() == 0) {
         io->mti = rng.mti;
@@ -310,101 +304,31 @@ __device__ __forceinline__ void seq_step(const chromo_replica *R, SeqIO *io, uin
     wave_sync();
 }
 
-// One (position, corner) term of the per-voxel fold (fields.pyx:1491-1520).
+// The proposal of slot S from its stored draws: the arithmetic of mc_step_dev's
+// proposal part (move_funcs.pyx).  False when a crank-shaft's axis beads
+// coincide: the reference then draws a random axis, which only the one-at-a-time
+// path can do in stream order.
 template <int NB>
-__device__ __forceinline__ void fold_term(int l, const double *wq, double pre, int32_t sq, double vol_bin,
-                                          double *acc, int &first, int ebase)
+__device__ __forceinline__ bool slot_proposal(const chromo_replica *R, const SpecSlot &S, Proposal &P)
 {
-    const double a = (l & 1) ? (1 - wq[0]) : wq[0];
-    const double b = (l & 2) ? (1 - wq[1]) : wq[1];
-    const double c = (l & 4) ? (1 - wq[2]) : wq[2];
-    const double wt = a * b * c;
-    const double c0 = wt / vol_bin;
-    const double t0 = pre * c0;
-    if (fabs(t0) > 1E-18) acc[0] += t0;
-#pragma unroll
-    for (int m = 1; m <= NB; ++m) {
-        const int st = (sq >> (8 * (m - 1))) & 0xff;
-        const double t = pre * (c0 * (double)st);
-        if (fabs(t) > 1E-18) acc[m] += t;
-    }
-    if (first < 0) first = ebase + l;
-}
-
-__device__ __forceinline__ bool axis_hit(int c, int d, int64_t nn)
-{
-    return nn == 1 ? true : (c == 0 ? d == 0 : d == 1);
-}
-
-__device__ __forceinline__ int64_t pack_ix(const int64_t ix[3])
-{
-    return ix[0] | (ix[1] << 21) | (ix[2] << 42);
-}
-
-// Evaluates one proposal (n <= SPEC_NMAX beads) with one wavefront and no LDS
-// arena: dE, the decision, the footprint, and the trial state kept in registers.
-// Lane i owns bead i; lane l owns contributions e = l and e = l + 64 (bead e/16,
-// configuration (e/8)%2, corner e%8) and folds, for the voxel of each, every
-// contribution in contribution order.  The per-voxel delta densities are the
-// reference's left folds bit for bit, and lanes sum their voxels' terms in the
-// same order as move_body, so dE equals the one-at-a-time path's.
-template <int NB>
-__device__ __forceinline__ void eval_small(const chromo_replica *R, const SpecSlot &S, SpecRes &res, SpecKeep &K,
-                           const SpecLds &L, int slot)
-{
-    constexpr int Bp1 = NB + 1;
     const int lane = lane_id();
     const int64_t N = R->N;
     const double *Pp = R->pos;
     const int mv = S.move;
     const int64_t n = S.n, ind0 = S.ind0, indf = S.indf;
-    const bool tangent = mv == CHROMO_TANGENT_ROTATION;
-    const bool state_change = mv == CHROMO_CHANGE_BINDING_STATE;
-    const bool check_field = R->field_on && !tangent;
-    const bool act = lane < n;
-    K.own[0] = K.own[1] = 0;
-    K.ins[0] = K.ins[1] = 0;
-    if (lane == 0) {
-        res.cmask = 0;
-        res.resample = 0;
-        res.overflow = 0;
-        res.nbins = 0;
-        res.rlo = ind0 - 1;
-        res.rhi = indf;
-        res.wlo = ind0;
-        res.whi = indf;
-    }
-    int64_t b = ind0 + lane;
-    if (tangent) b = act ? S.picks[lane] : 0;
-    if (!act) b = tangent ? 0 : ind0;
-    K.b = b;
-
-    double o[9];
-    int8_t so[NB], sn[NB];
+    P.move = mv;
+    P.n = n;
+    P.ind0 = ind0;
+    P.indf = indf;
+    P.binder = S.binder;
+    P.mypick = 0;
+    P.mynew = 0;
 #pragma unroll
-    for (int c = 0; c < 9; ++c) o[c] = 0;
+    for (int q = 0; q < 12; ++q) P.T[q] = 0;
 #pragma unroll
-    for (int m = 0; m < NB; ++m) so[m] = sn[m] = 0;
-    if (act) {
-#pragma unroll
-        for (int c = 0; c < 9; ++c) o[c] = Pp[c * N + b];
-#pragma unroll
-        for (int m = 0; m < NB; ++m) {
-            so[m] = R->states[(int64_t)m * N + b];
-            sn[m] = so[m];
-        }
-        if (state_change) {
-#pragma unroll
-            for (int m = 0; m < NB; ++m)
-                if (m == S.binder) sn[m] = (int8_t)S.newst[lane];
-        }
-    }
-
-    // transformation (move_funcs.pyx: crank 157-280, end pivot 349-398, slide 440-444)
-    double T[12];
-#pragma unroll
-    for (int q = 0; q < 12; ++q) T[q] = 0;
+    for (int q = 0; q < 9; ++q) P.myT[q] = 0;
     if (mv == CHROMO_CRANK_SHAFT) {
+        // get_crank_shaft_axis / _fulcrum, move_funcs.pyx:157-280
         int64_t ia, ib;
         const int64_t indf_m1 = indf - 1, ind0_m1 = ind0 - 1;
         if (ind0 == indf_m1 && ind0 == 0) { ia = indf; ib = ind0; }
@@ -424,14 +348,12 @@ __device__ __forceinline__ void eval_small(const chromo_replica *R, const SpecSl
         load_pos3(Pp, N, ifc, pt);
         for (int i = 0; i < 3; ++i) dir[i] = ra[i] - rb[i];
         double mag = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-        if (mag < 1E-5) {
-            if (lane == 0) res.resample = 1;
-            return;
-        }
+        if (mag < 1E-5) return false;
         double sc = 1.0 / mag;
         for (int i = 0; i < 3; ++i) dir[i] = dir[i] * sc;
-        axis_rotation(dir, pt, S.ang, T);
+        axis_rotation(dir, pt, S.ang, P.T);
     } else if (mv == CHROMO_END_PIVOT) {
+        // get_end_pivot_fulcrum, move_funcs.pyx:349-398
         int64_t ifc;
         if (ind0 == 0 && indf != N) ifc = indf;
         else if (ind0 != 0 && indf == N) ifc = ind0 - 1;
@@ -440,282 +362,130 @@ __device__ __forceinline__ void eval_small(const chromo_replica *R, const SpecSl
         double pt[3], axis[3];
         load_pos3(Pp, N, ifc, pt);
         sphere_from(S.s1, S.s2, axis);
-        axis_rotation(axis, pt, S.ang, T);
+        axis_rotation(axis, pt, S.ang, P.T);
     } else if (mv == CHROMO_SLIDE) {
         double dir[3];
         sphere_from(S.s1, S.s2, dir);
         for (int i = 0; i < 3; ++i) dir[i] *= S.ang;
-        T[3] = dir[0];
-        T[7] = dir[1];
-        T[11] = dir[2];
-    } else if (tangent && act) {
-        double dir[3];
-        const double origin[3] = {0., 0., 0.};
-        sphere_from(S.ps1[lane], S.ps2[lane], dir);
-        axis_rotation(dir, origin, S.ang, T);
-    }
-    double w[9];
-    if (mv == CHROMO_CRANK_SHAFT || mv == CHROMO_END_PIVOT) {
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            w[j] = rowdot(&T[4 * j], o[0], o[1], o[2]) + T[4 * j + 3];
-            w[3 + j] = rowdot(&T[4 * j], o[3], o[4], o[5]);
-            w[6 + j] = rowdot(&T[4 * j], o[6], o[7], o[8]);
-        }
-    } else if (mv == CHROMO_SLIDE) {
-#pragma unroll
-        for (int j = 0; j < 3; ++j) w[j] = o[j] + T[4 * j + 3];
-#pragma unroll
-        for (int c = 3; c < 9; ++c) w[c] = o[c];
-    } else if (tangent) {
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            w[j] = o[j];
-            w[3 + j] = rowdot(&T[4 * j], o[3], o[4], o[5]);
-            w[6 + j] = rowdot(&T[4 * j], o[6], o[7], o[8]);
+        P.T[3] = dir[0];
+        P.T[7] = dir[1];
+        P.T[11] = dir[2];
+    } else if (mv == CHROMO_TANGENT_ROTATION) {
+        if (lane < n) {
+            double dir[3], T[12];
+            const double origin[3] = {0., 0., 0.};
+            P.mypick = S.picks[lane];
+            sphere_from(S.ps1[lane], S.ps2[lane], dir);
+            axis_rotation(dir, origin, S.ang, T);
+            for (int j = 0; j < 3; ++j)
+                for (int q = 0; q < 3; ++q) P.myT[3 * j + q] = T[4 * j + q];
         }
     } else {
-#pragma unroll
-        for (int c = 0; c < 9; ++c) w[c] = o[c];
+        if (lane < n) P.mynew = S.newst[lane];
     }
+    return true;
+}
 
-    // polymer dE (SSWLC.compute_dE, polymers.pyx:1024-1082)
-    double dEp = 0;
-    if (tangent || state_change) {
-        double e = 0;
-        if (act && tangent) {
-            if (b != 0) {
-                double r0[3], t30[3];
-                for (int j = 0; j < 3; ++j) {
-                    r0[j] = Pp[j * N + b - 1];
-                    t30[j] = Pp[(3 + j) * N + b - 1];
-                }
-                Bond bd = load_bond(R->bonds, N, b - 1);
-                e += pair_dE_forward(bd, r0, &o[0], &w[0], t30, &o[3], &w[3]);
-            }
-            if (b + 1 != N) {
-                double r1[3], t31[3];
-                for (int j = 0; j < 3; ++j) {
-                    r1[j] = Pp[j * N + b + 1];
-                    t31[j] = Pp[(3 + j) * N + b + 1];
-                }
-                Bond bd = load_bond(R->bonds, N, b);
-                e += pair_dE_reverse(bd, &o[0], &w[0], r1, &o[3], &w[3], t31);
-            }
-        } else if (act) {
-            int8_t mods[NB];
-#pragma unroll
-            for (int m = 0; m < NB; ++m) mods[m] = R->marks[(int64_t)m * N + b];
-            e = bead_binding_dE<NB>(R, mods, so, sn);
-        }
-        for (int i = 0; i < n; ++i) dEp += readlane_f64(e, i);
-    } else {
-        double ef = 0, er = 0;
-        if (lane == 0 && ind0 != 0) {
-            double r0[3], t30[3];
-            for (int j = 0; j < 3; ++j) {
-                r0[j] = Pp[j * N + ind0 - 1];
-                t30[j] = Pp[(3 + j) * N + ind0 - 1];
-            }
-            Bond bd = load_bond(R->bonds, N, ind0 - 1);
-            ef = pair_dE_forward(bd, r0, &o[0], &w[0], t30, &o[3], &w[3]);
-        }
-        if (lane == n - 1 && indf != N) {
-            double r1[3], t31[3];
-            for (int j = 0; j < 3; ++j) {
-                r1[j] = Pp[j * N + indf];
-                t31[j] = Pp[(3 + j) * N + indf];
-            }
-            Bond bd = load_bond(R->bonds, N, indf - 1);
-            er = pair_dE_reverse(bd, &o[0], &w[0], r1, &o[3], &w[3], t31);
-        }
-        if (ind0 != 0) dEp += readlane_f64(ef, 0);
-        if (indf != N) dEp += readlane_f64(er, (int)(n - 1));
+// Wavefront `slot` evaluates slot S into res, keeps the trial state in K and in
+// its evaluator block, and publishes the voxels it reads to the conflict table
+// (their table slots replace the first-position entries of G.uaux).  Returns
+// whether it inserted into the table.
+template <int NB>
+__device__ __forceinline__ bool eval_slot(const chromo_replica *R, const SpecSlot &S, SpecRes &res, SegKeep<NB> &K,
+                                          const SpecLds &L, const SegLds &G, int slot)
+{
+    const int lane = lane_id();
+    const int64_t ind0 = S.ind0, indf = S.indf;
+    if (lane == 0) {
+        res.cmask = 0;
+        res.resample = 0;
+        res.overflow = 0;
+        res.nbins = 0;
+        res.rlo = ind0 - 1;
+        res.rhi = indf;
+        res.wlo = ind0;
+        res.whi = indf;
     }
-
-    // field dE (UniformDensityField.compute_dE, fields.pyx:1149-1233)
-    double dEf = 0;
-    int U = 0;
-    if (check_field) {
-        int64_t P0 = 0, P1 = 0;
-        double wt0[3] = {0, 0, 0}, wt1[3] = {0, 0, 0};
-        int32_t st0 = 0, st1 = 0;
-        if (act) {
-            int64_t ix[3];
-            voxel_lower(R, &o[0], ix, wt0);
-            P0 = pack_ix(ix);
-            double pt[3];
-            for (int j = 0; j < 3; ++j) pt[j] = state_change ? o[j] : w[j];
-            voxel_lower(R, pt, ix, wt1);
-            P1 = pack_ix(ix);
-#pragma unroll
-            for (int m = 0; m < NB; ++m) {
-                st0 |= ((int32_t)(uint8_t)so[m]) << (8 * m);
-                st1 |= ((int32_t)(uint8_t)(state_change ? sn[m] : so[m])) << (8 * m);
-            }
-        }
-        const int64_t nx = R->nx, ny = R->ny, nz = R->nz;
-        const int E = (int)(16 * n);
-        int X[2], Y[2], Z[2], first[2], valid[2];
-        int32_t vox[2];
-        double rho[2][Bp1], acc[2][Bp1];
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int e = lane + WAVE * j;
-            valid[j] = e < E;
-            const int p = e >> 3, l = e & 7;
-            const int i = (p >> 1) & (WAVE - 1), k = p & 1;
-            const int64_t q0 = __shfl(P0, i, WAVE);
-            const int64_t q1 = __shfl(P1, i, WAVE);
-            const int64_t q = k ? q1 : q0;
-            int64_t x = (q & 0x1fffff) + (l & 1), y = ((q >> 21) & 0x1fffff) + ((l >> 1) & 1),
-                    z = ((q >> 42) & 0x1fffff) + ((l >> 2) & 1);
-            if (x >= nx) x -= nx;
-            if (y >= ny) y -= ny;
-            if (z >= nz) z -= nz;
-            X[j] = (int)x;
-            Y[j] = (int)y;
-            Z[j] = (int)z;
-            vox[j] = (int32_t)(x + y * nx + z * nx * ny);
-            first[j] = -1;
-#pragma unroll
-            for (int m = 0; m < Bp1; ++m) {
-                acc[j][m] = 0;
-                rho[j][m] = valid[j] ? R->density[(int64_t)vox[j] * Bp1 + m] : 0.0;
-            }
-        }
-        const bool degen = nx == 1 || ny == 1 || nz == 1;
-        const double vol_bin = R->vol_bin;
-        for (int pq = 0; pq < 2 * n; ++pq) {
-            const int iq = pq >> 1, kq = pq & 1;
-            const int64_t qp = readlane_i64(kq ? P1 : P0, iq);
-            double wq[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) wq[c] = readlane_f64(kq ? wt1[c] : wt0[c], iq);
-            const int32_t sq = (int32_t)readlane_u32((uint32_t)(kq ? st1 : st0), iq);
-            const double pre = kq ? 1. : -1.;
-            const int qx = (int)(qp & 0x1fffff), qy = (int)((qp >> 21) & 0x1fffff),
-                      qz = (int)((qp >> 42) & 0x1fffff);
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                if (!valid[j]) continue;
-                int dx = X[j] - qx, dy = Y[j] - qy, dz = Z[j] - qz;
-                if (dx < 0) dx += (int)nx;
-                if (dy < 0) dy += (int)ny;
-                if (dz < 0) dz += (int)nz;
-                if (!degen) {
-                    if (dx <= 1 && dy <= 1 && dz <= 1)
-                        fold_term<NB>(dx | (dy << 1) | (dz << 2), wq, pre, sq, vol_bin, acc[j], first[j],
-                                      8 * pq);
-                } else {
-                    for (int l = 0; l < 8; ++l)
-                        if (axis_hit(l & 1, dx, nx) && axis_hit((l >> 1) & 1, dy, ny) &&
-                            axis_hit((l >> 2) & 1, dz, nz))
-                            fold_term<NB>(l, wq, pre, sq, vol_bin, acc[j], first[j], 8 * pq);
-                }
-            }
-        }
-        double S2[NB], SX[NB * NB], NS = 0;
-#pragma unroll
-        for (int j = 0; j < NB; ++j) S2[j] = 0;
-#pragma unroll
-        for (int j = 0; j < NB * NB; ++j) SX[j] = 0;
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            K.own[j] = valid[j] && first[j] == lane + WAVE * j;
-            if (K.own[j]) voxel_dE_terms<NB>(R, rho[j], acc[j], S2, SX, NS);
-            K.vox[j] = vox[j];
-#pragma unroll
-            for (int m = 0; m < Bp1; ++m) K.nr[j][m] = rho[j][m] + acc[j][m];
-        }
-        const double share = wave_sum(lane_field_share<NB>(R, S2, SX, NS));
-        U = __popcll(__ballot(K.own[0])) + __popcll(__ballot(K.own[1]));
-        // count_doubly_bound (fields.pyx:1877-1937) and confinement (:121-202)
-        int dbl_t[NB], dbl[NB];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            dbl_t[j] = __popcll(__ballot(act && (state_change ? sn[j] : so[j]) == 2));
-            dbl[j] = __popcll(__ballot(act && so[j] == 2));
-        }
-        int out_t = 0, out_c = 0;
-        const int ct = R->confine_type;
-        if (ct != 0 && !state_change) {
-            const double cl = R->confine_length;
-            if (ct == 1) {
-                out_t = __popcll(__ballot(act && sqrt(dot3(&w[0], &w[0])) > cl));
-                out_c = __popcll(__ballot(act && sqrt(dot3(&o[0], &o[0])) > cl));
-            } else {
-#pragma unroll
-                for (int j = 0; j < 3; ++j) out_t += __popcll(__ballot(act && fabs(w[j]) > cl / 2));
-            }
-        }
-        dEf = field_dE_total<NB>(R, share, dbl_t, dbl, out_t, out_c, state_change);
-
-        // publish the voxels to the conflict table
+    Proposal P;
+    if (!slot_proposal<NB>(R, S, P)) {
+        if (lane == 0) res.resample = 1;
+        wave_sync();
+        return false;
+    }
+    SegRes sr;
+    const double u = (double)S.ud / RMAX;
+    seg_eval<NB>(R, P, u, G, sr, K);
+    if (sr.overflow) {
+        // more voxels than seg_eval holds: the one-at-a-time path evaluates it
+        if (lane == 0) res.overflow = 1;
+        wave_sync();
+        return false;
+    }
+    const int U = sr.U;
+    bool inserted = false;
+    int full = 0;
+    if (U > 0) {
         int base = 0;
         if (lane == 0) base = atomicAdd(&L.ctl->used, U);
         base = bcast0(base);
         if (slot > 0 && base + U > SPEC_HASH_SOFT) {
-            if (lane == 0) res.overflow = 1;
+            full = 1;
         } else {
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                if (!K.own[j]) continue;
-                uint32_t h = ((uint32_t)vox[j] * 2654435761u) >> SPEC_HASH_SHIFT;
+            inserted = true;
+            for (int q = lane; q < U; q += WAVE) {
+                const int32_t v = G.uvox[q];
+                uint32_t h = ((uint32_t)v * 2654435761u) >> SPEC_HASH_SHIFT;
                 int probes = 0;
-                bool full = false;
                 for (;;) {
-                    int old = atomicCAS(&L.hkey[h], HKEY_EMPTY, vox[j]);
-                    if (old == HKEY_EMPTY || old == vox[j]) break;
+                    int old = atomicCAS(&L.hkey[h], HKEY_EMPTY, v);
+                    if (old == HKEY_EMPTY || old == v) break;
                     h = (h + 1) & (uint32_t)(SPEC_HASH - 1);
                     if (++probes >= SPEC_HASH) {
-                        full = true;
+                        full = 1;
                         break;
                     }
                 }
-                if (full) {
-                    res.overflow = 1;
+                if (probes >= SPEC_HASH) {
+                    G.uaux[q] = -1;
                     continue;
                 }
                 atomicOr(&L.hmask[h], 1u << slot);
-                K.hs[j] = (int32_t)h;
-                K.ins[j] = 1;
+                G.uaux[q] = (int32_t)h;
             }
         }
     }
-
-    // Metropolis (mc_sim.pyx:152-182)
-    double dE = 0;
-    dE += dEp;
-    if (check_field) dE += dEf;
-    const double expdE = exp(-dE);
-    const double u = (double)S.ud / RMAX;
-    const int accepted = u < expdE;
-#pragma unroll
-    for (int c = 0; c < 9; ++c) K.w[c] = w[c];
+    full = __any(full);
     if (lane == 0) {
-        res.dEp = dEp;
-        res.dEf = dEf;
+        res.dEp = sr.dEp;
+        res.dEf = sr.dEf;
         res.u = u;
-        res.expdE = expdE;
-        res.accepted = accepted;
+        res.expdE = sr.expdE;
+        res.accepted = sr.accepted;
         res.nbins = U;
+        res.overflow = full;
     }
+    wave_sync();
+    return inserted;
 }
 
-// Earlier slots of this round whose writes slot s reads.
-__device__ __forceinline__ void spec_conflicts(const SpecSlot *slots, SpecRes *res, const SpecKeep &K, const SpecLds &L,
-                               int s)
+// Earlier slots of this round whose writes slot s reads: shared voxels through
+// the conflict table, shared beads through the bead ranges (tangent rotation:
+// picks within one bead of each other).
+__device__ __forceinline__ void spec_conflicts(const SpecSlot *slots, SpecRes *res, const SpecLds &L,
+                                               const SegLds &G, int s)
 {
+    static_assert((SPEC_WMAX - 1) * SPEC_NMAX <= WAVE, "one lane per (earlier slot, pick)");
     const int lane = lane_id();
     const uint32_t low = (1u << s) - 1u;
     uint32_t m = 0;
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-        if (K.ins[j]) m |= L.hmask[K.hs[j]] & low;
+    const int U = res[s].nbins;
+    for (int q = lane; q < U; q += WAVE) {
+        const int h = G.uaux[q];
+        if (h >= 0) m |= L.hmask[h] & low;
+    }
     const SpecSlot &S = slots[s];
     if (S.move == CHROMO_TANGENT_ROTATION) {
-        const int a = lane >> 3, c = lane & 7;
+        const int a = lane / SPEC_NMAX, c = lane % SPEC_NMAX;
         if (a < s && slots[a].kind == SLOT_SMALL && c < S.n) {
             const int64_t mine = S.picks[c];
             for (int64_t q = 0; q < slots[a].n; ++q) {
@@ -731,28 +501,17 @@ __device__ __forceinline__ void spec_conflicts(const SpecSlot *slots, SpecRes *r
     if (lane == 0) res[s].cmask = m;
 }
 
-template <int NB>
-__device__ __forceinline__ void spec_commit(const chromo_replica *R, const SpecSlot &S, const SpecKeep &K)
+// Clears this wavefront's conflict-table entries (after every wavefront has read
+// the table this round).
+__device__ __forceinline__ void spec_release(const SpecLds &L, const SegLds &G, int U)
 {
-    constexpr int Bp1 = NB + 1;
-    const int lane = lane_id();
-    const int64_t N = R->N;
-    if (lane < S.n) {
-        if (S.move == CHROMO_CHANGE_BINDING_STATE) {
-            R->states[(int64_t)S.binder * N + K.b] = (int8_t)S.newst[lane];
-        } else {
-            const int c0 = (S.move == CHROMO_TANGENT_ROTATION) ? 3 : 0;
-            const int c1 = (S.move == CHROMO_SLIDE) ? 3 : 9;
-#pragma unroll
-            for (int c = 0; c < 9; ++c)
-                if (c >= c0 && c < c1) R->pos[c * N + K.b] = K.w[c];
+    for (int q = lane_id(); q < U; q += WAVE) {
+        const int h = G.uaux[q];
+        if (h >= 0) {
+            L.hkey[h] = HKEY_EMPTY;
+            L.hmask[h] = 0;
         }
     }
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-        if (K.own[j])
-#pragma unroll
-            for (int m = 0; m < Bp1; ++m) R->density[(int64_t)K.vox[j] * Bp1 + m] = K.nr[j][m];
 }
 
 __device__ __forceinline__ void slot_copy(SpecSlot &dst, const SpecSlot &src)
@@ -903,9 +662,9 @@ __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng
                     io->C = C;
                 }
                 wave_sync();
-                seq_step<NB, W>(R, io, L.mt);
+                seq_step<NB, W>(R, io, L.mt, L.first);
                 rng_restore(rng, io->ck);
-                rng.mti = io->mti;
+                rng.mti = uni_i32(io->mti);
                 if (lane == 0) C = io->C;
                 wave_sync();
                 const MoveOut o = io->out;
@@ -918,11 +677,12 @@ __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng
                 nq = 0;
             }
         } else {
-            SpecKeep keep;
+            SegKeep<NB> keep;
             const bool mine = wv < cnt && cs[wv].kind == SLOT_SMALL;
-            if (mine) eval_small<NB>(R, cs[wv], L.res[wv], keep, L, wv);
+            bool inserted = false;
+            if (mine) inserted = eval_slot<NB>(R, cs[wv], L.res[wv], keep, L, L.mine, wv);
             __syncthreads();
-            if (mine && !L.res[wv].resample && !L.res[wv].overflow) spec_conflicts(cs, L.res, keep, L, wv);
+            if (mine && !L.res[wv].resample && !L.res[wv].overflow) spec_conflicts(cs, L.res, L, L.mine, wv);
             __syncthreads();
             // resolution, identical in every wavefront
             int F = cnt, rew = 0;
@@ -943,15 +703,12 @@ __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng
                 }
                 if (q.accepted) accm |= 1u << s;
             }
-            if (mine && wv < F && L.res[wv].accepted) spec_commit<NB>(R, cs[wv], keep);
-            if (mine) {
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    if (keep.ins[j]) {
-                        L.hkey[keep.hs[j]] = HKEY_EMPTY;
-                        L.hmask[keep.hs[j]] = 0;
-                    }
+            if (mine && wv < F && L.res[wv].accepted) {
+                const SpecSlot &S = cs[wv];
+                const int mynew = lane < S.n ? S.newst[lane] : 0;
+                seg_commit<NB>(R, S.move, S.n, S.binder, mynew, keep, L.mine, L.res[wv].nbins);
             }
+            if (inserted) spec_release(L, L.mine, L.res[wv].nbins);
             if (wv == 0) {
                 spec_book(R, C, K, cs, L.res, F);
                 nq = spec_carry(cs, ns, F, nqa, rew);
@@ -968,7 +725,11 @@ __device__ __forceinline__ void spec_run(const chromo_replica *R, int64_t steps)
 {
     constexpr int Bp1 = NB + 1;
     const SpecLayout SL = spec_layout(Bp1, W);
+    const int wv = wave_id(), lane = lane_id();
     SpecLds L;
+    L.mine = seg_lds(lds_dyn + SL.seg + SL.seg_bytes * wv, Bp1);
+    L.first = seg_lds(lds_dyn + SL.seg, Bp1);
+    seg_clear(L.mine);
     L.slots[0] = (SpecSlot *)(lds_dyn + SL.slots);
     L.slots[1] = L.slots[0] + W;
     L.res = (SpecRes *)(lds_dyn + SL.res);
@@ -979,7 +740,6 @@ __device__ __forceinline__ void spec_run(const chromo_replica *R, int64_t steps)
     L.mt = (uint32_t *)(lds_dyn + SL.mt);
     __shared__ chromo_ctrl sctrl[MAX_CTRL];
     __shared__ Counters sK;
-    const int wv = wave_id(), lane = lane_id();
     for (int i = (int)threadIdx.x; i < SPEC_HASH; i += WAVE * W) {
         L.hkey[i] = HKEY_EMPTY;
         L.hmask[i] = 0;
@@ -988,7 +748,6 @@ __device__ __forceinline__ void spec_run(const chromo_replica *R, int64_t steps)
     WaveRng rng;
     bool need_mt = false;
     if (wv == 0) {
-        lds_hash_clear(Bp1);
         need_mt = any_binding(R);
         rng_load(R, rng, L.mt, need_mt);
         if (lane == 0) {

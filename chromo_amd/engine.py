@@ -149,6 +149,7 @@ class DeviceReplicas:
     """Chains, fields, controllers and RNG streams of ``n`` replicas in HBM."""
 
     CTRL_SLOTS = 8
+    SEG_MAX = 32       # beads per move of the one-wavefront evaluator (chromo_seg.hpp)
     WAVE_CHOICES = (1, 2)
     waves = 1          # wavefronts per replica used by run(); see chromo_amd.autotune
 
@@ -182,6 +183,8 @@ class DeviceReplicas:
                 raise NotImplementedError(f"chemical_mods values outside 0..{L.MAX_MARK - 1}")
             if f.field_on and f.n_bins >= 2 ** 31:
                 raise NotImplementedError("more than 2^31 voxels")
+            if f.field_on and max(int(f.nx), int(f.ny), int(f.nz)) > 32767:
+                raise NotImplementedError("more than 32767 voxels along one axis")
         self.Ns, self.Bs = Ns, Bs
         self.binder_mask = sum(1 << b for b in set(Bs))
         nctrl = [len(c) for c in ctrls]
@@ -190,8 +193,12 @@ class DeviceReplicas:
         self.nctrl = list(nctrl)
         if caps is None:
             caps = [default_caps(c, N) for c, N in zip(ctrls, Ns)]
-        self.bead_cap = [int(c[0]) for c in caps]
-        self.seg_cap = [min(int(c[1]), int(c[0])) for c in caps]
+        # every replica gets a global arena of at least min(N, 32) beads: moves of up
+        # to 32 beads fall back to it when they touch more voxels than the
+        # registers of the one-wavefront evaluator hold
+        self.bead_cap = [max(int(c[0]), min(N, self.SEG_MAX)) for c, N in zip(caps, Ns)]
+        self.seg_cap = [min(max(int(c[1]), min(N, self.SEG_MAX)), bc)
+                        for c, N, bc in zip(caps, Ns, self.bead_cap)]
 
         # offsets into flat buffers
         def offsets(sizes):
@@ -206,7 +213,7 @@ class DeviceReplicas:
         self.nbins = nbins
         self.o_den = offsets([nb * (B + 1) for nb, B in zip(nbins, Bs)])
         self.o_ctrl = offsets([self.CTRL_SLOTS] * n)
-        sbytes = [int(self.lib.chromo_scratch_bytes(bc, sc, B)) if bc > 32 else 0
+        sbytes = [int(self.lib.chromo_scratch_bytes(bc, sc, B))
                   for bc, sc, B in zip(self.bead_cap, self.seg_cap, Bs)]
         if any(b < 0 for b in sbytes):
             raise ValueError("invalid arena capacities")
