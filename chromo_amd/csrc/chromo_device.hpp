@@ -90,6 +90,22 @@ __device__ __forceinline__ double readlane_f64(double v, int l)
 
 __device__ __forceinline__ double bcast0(double x) { return readlane_f64(x, 0); }
 
+// The same pointer in the global address space: its loads and stores are global_*
+// instructions, counted apart from LDS and scalar loads, so waiting for an LDS
+// value does not also wait for outstanding HBM traffic (generic flat_*
+// instructions count on both).
+#define GLOBAL_AS __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ GLOBAL_AS T *gmem(T *p)
+{
+    return (GLOBAL_AS T *)p;
+}
+
+// True when pred holds on any active lane.  A ballot, which the compiler knows is
+// wave-uniform (a loop that exits on it keeps its other values in scalar
+// registers); the runtime's __any is an opaque library call.
+__device__ __forceinline__ bool wave_any(bool pred) { return __builtin_amdgcn_ballot_w64(pred) != 0; }
+
 // A value every lane holds equally, marked as such for the compiler (scalar
 // registers and scalar arithmetic).
 __device__ __forceinline__ uint32_t uni_u32(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
@@ -215,14 +231,21 @@ struct WaveRng {
     int gf, gr;
     uint32_t *mt;       // 624 words in LDS (numpy MT19937 state)
     int mti;
+    // Philox: a batch of WAVE consecutive blocks, block pbase + j on lane j (words
+    // p0..p3); pidx is the next of its 4 * WAVE words.  pbase == PHILOX_NONE: no
+    // batch yet.
     uint32_t k0, k1;
-    uint64_t ctr;
-    uint32_t b0, b1, b2, b3;
-    int bi;
+    uint64_t pbase;
+    uint32_t p0, p1, p2, p3;
+    int pidx;
 };
+
+constexpr uint64_t PHILOX_NONE = ~0ull;
 
 __device__ __forceinline__ int32_t glibc_draw(WaveRng &R)
 {
+    R.gf = uni_i32(R.gf);
+    R.gr = uni_i32(R.gr);
     uint32_t a = readlane_u32(R.g, R.gf);
     uint32_t b = readlane_u32(R.g, R.gr);
     uint32_t v = a + b;
@@ -256,21 +279,69 @@ __device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1
     }
 }
 
+// Blocks base .. base + WAVE - 1 of the stream, one per lane: the same words a
+// block-at-a-time generator produces, WAVE blocks per Philox latency.
+__device__ __forceinline__ void philox_fill(WaveRng &R, uint64_t base)
+{
+    R.pbase = uni_u64(base);
+    const uint64_t c = R.pbase + (uint64_t)lane_id();
+    uint32_t w[4] = {(uint32_t)c, (uint32_t)(c >> 32), 0u, 0u};
+    philox10(w, uni_u32(R.k0), uni_u32(R.k1));
+    R.p0 = w[0];
+    R.p1 = w[1];
+    R.p2 = w[2];
+    R.p3 = w[3];
+}
+
 __device__ __forceinline__ uint32_t philox_draw(WaveRng &R)
 {
-    if (R.bi >= 4) {
-        uint32_t c[4] = {(uint32_t)R.ctr, (uint32_t)(R.ctr >> 32), 0u, 0u};
-        philox10(c, R.k0, R.k1);
-        R.b0 = c[0];
-        R.b1 = c[1];
-        R.b2 = c[2];
-        R.b3 = c[3];
-        R.ctr++;
-        R.bi = 0;
+    R.pidx = uni_i32(R.pidx);
+    if (R.pidx >= 4 * WAVE) {
+        philox_fill(R, R.pbase + WAVE);
+        R.pidx = 0;
     }
-    uint32_t v = R.bi == 0 ? R.b0 : (R.bi == 1 ? R.b1 : (R.bi == 2 ? R.b2 : R.b3));
-    R.bi++;
+    const int w = R.pidx & 3;
+    const uint32_t x = w == 0 ? R.p0 : (w == 1 ? R.p1 : (w == 2 ? R.p2 : R.p3));
+    const uint32_t v = readlane_u32(x, R.pidx >> 2);
+    R.pidx++;
     return v;
+}
+
+// The block-at-a-time form of the Philox position, as stored between launches and
+// in checkpoints: the next block to generate (ctr), the next word of the current
+// block (bi, 4 when it is used up) and that block's words.
+__device__ __forceinline__ void philox_get(const WaveRng &R, uint64_t &ctr, int &bi, uint32_t b[4])
+{
+    const uint64_t blk = R.pbase + (uint64_t)(R.pidx >> 2);   // block of the next draw
+    if ((R.pidx & 3) == 0) {
+        ctr = blk;
+        bi = 4;
+        b[0] = b[1] = b[2] = b[3] = 0;
+    } else {
+        const int l = R.pidx >> 2;
+        ctr = blk + 1;
+        bi = R.pidx & 3;
+        b[0] = readlane_u32(R.p0, l);
+        b[1] = readlane_u32(R.p1, l);
+        b[2] = readlane_u32(R.p2, l);
+        b[3] = readlane_u32(R.p3, l);
+    }
+}
+
+// Moves to the position (ctr, bi) of philox_get; recomputes the batch only when
+// that position lies outside it.
+__device__ __forceinline__ void philox_set(WaveRng &R, uint64_t ctr, int bi)
+{
+    ctr = uni_u64(ctr);
+    bi = uni_i32(bi);
+    const uint64_t blk = bi >= 4 ? ctr : ctr - 1;
+    const int w = bi >= 4 ? 0 : bi;
+    if (R.pbase == PHILOX_NONE || blk < R.pbase || blk >= R.pbase + WAVE) {
+        philox_fill(R, blk);
+        R.pidx = w;
+    } else {
+        R.pidx = (int)(blk - R.pbase) * 4 + w;
+    }
 }
 
 // MT19937 regeneration of the 624-word block (lane 0, LDS), then tempering.
@@ -292,6 +363,7 @@ __device__ void mt_twist(uint32_t *mt)
 
 __device__ __forceinline__ uint32_t mt_draw(WaveRng &R)
 {
+    R.mti = uni_i32(R.mti);
     if (R.mti >= 624) {
         mt_twist(R.mt);
         R.mti = 0;
@@ -498,8 +570,9 @@ struct Bond {
     double eb, ep, eperp, gam, eta;
 };
 
-__device__ __forceinline__ Bond load_bond(const double *bonds, int64_t N, int64_t b)
+__device__ __forceinline__ Bond load_bond(const double *bonds_, int64_t N, int64_t b)
 {
+    const GLOBAL_AS double *bonds = gmem(bonds_);
     const int64_t M = N - 1;
     Bond o;
     o.eb = bonds[b];

@@ -24,7 +24,7 @@ hipError_t spec_launch_w2(const chromo_replica *, int, int64_t, int, hipStream_t
 namespace {
 
 // Persistent mc_sim: one workgroup (one wave) per replica.
-__global__ void __launch_bounds__(64) k_mc_run(const chromo_replica *reps, int n_reps, int64_t steps,
+__global__ void __launch_bounds__(64) k_mc_run(const chromo_replica *__restrict__ reps, int n_reps, int64_t steps,
                                                int max_bp1)
 {
     const int r = blockIdx.x;
@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(64) k_mc_run(const chromo_replica *reps, int n
 }
 
 // One mc_step per launch (the plain path / mc_step API).
-__global__ void __launch_bounds__(64) k_mc_step(const chromo_replica *reps, int rep, int ci,
+__global__ void __launch_bounds__(64) k_mc_step(const chromo_replica *__restrict__ reps, int rep, int ci,
                                                 int max_bp1)
 {
     const chromo_replica *R = &reps[rep];

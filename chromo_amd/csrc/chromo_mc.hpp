@@ -373,7 +373,7 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, double u, Pro
             atomicMin(&A.hmin[h], e);
             A.eslot[e] = (int)h;
         }
-        if (__any(bad)) {
+        if (wave_any(bad)) {
             out.err = ERR_ARENA;
             return;
         }
@@ -434,7 +434,7 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, double u, Pro
             if (pos >= 0 && pos < E) A.list[pos] = e;
             else outside = 1;
         }
-        if (__any(outside)) {
+        if (wave_any(outside)) {
             out.err = ERR_ARENA;
             return;
         }
@@ -589,8 +589,9 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, double u, Pro
 // per-bead phase.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void load_pos3(const double *Pp, int64_t N, int64_t b, double v[3])
+__device__ __forceinline__ void load_pos3(const double *Pp_, int64_t N, int64_t b, double v[3])
 {
+    const GLOBAL_AS double *Pp = gmem(Pp_);
     v[0] = Pp[b];
     v[1] = Pp[N + b];
     v[2] = Pp[2 * N + b];
@@ -719,7 +720,7 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
                 int64_t c = draw_mod(rnd31(rng), N);
                 int dup = (lane < i && P.mypick == c);
                 for (int64_t j = WAVE + lane; j < i; j += WAVE) dup |= (A.inds[j] == c);
-                if (!__any(dup)) {
+                if (!wave_any(dup)) {
                     if (i < WAVE) {
                         if (lane == i) P.mypick = c;
                     } else {
@@ -899,12 +900,12 @@ __device__ __forceinline__ void rng_load(const chromo_replica *R, WaveRng &rng, 
     }
     rng.k0 = uni_u32(w[CHROMO_RNG_PHILOX_KEY]);
     rng.k1 = uni_u32(w[CHROMO_RNG_PHILOX_KEY + 1]);
-    rng.ctr = uni_u64((uint64_t)w[CHROMO_RNG_PHILOX_CTR] | ((uint64_t)w[CHROMO_RNG_PHILOX_CTR + 1] << 32));
-    rng.b0 = uni_u32(w[CHROMO_RNG_PHILOX_BUF]);
-    rng.b1 = uni_u32(w[CHROMO_RNG_PHILOX_BUF + 1]);
-    rng.b2 = uni_u32(w[CHROMO_RNG_PHILOX_BUF + 2]);
-    rng.b3 = uni_u32(w[CHROMO_RNG_PHILOX_BUF + 3]);
-    rng.bi = uni_i32((int)w[CHROMO_RNG_PHILOX_IDX]);
+    rng.pbase = PHILOX_NONE;
+    rng.pidx = 0;
+    rng.p0 = rng.p1 = rng.p2 = rng.p3 = 0;
+    if (rng.mode == CHROMO_RNG_PHILOX)
+        philox_set(rng, (uint64_t)w[CHROMO_RNG_PHILOX_CTR] | ((uint64_t)w[CHROMO_RNG_PHILOX_CTR + 1] << 32),
+                   (int)w[CHROMO_RNG_PHILOX_IDX]);
     wave_sync();
 }
 
@@ -917,17 +918,21 @@ __device__ __forceinline__ void rng_store(const chromo_replica *R, const WaveRng
         wave_sync();
         for (int i = lane; i < 624; i += WAVE) w[CHROMO_RNG_MT_STATE + i] = rng.mt[i];
     }
+    uint64_t ctr = 0;
+    int bi = 4;
+    uint32_t b[4] = {0, 0, 0, 0};
+    const bool philox = rng.mode == CHROMO_RNG_PHILOX;
+    if (philox) philox_get(rng, ctr, bi, b);
     if (lane == 0) {
         w[CHROMO_RNG_GLIBC_F] = (uint32_t)rng.gf;
         w[CHROMO_RNG_GLIBC_R] = (uint32_t)rng.gr;
         w[CHROMO_RNG_MT_INDEX] = (uint32_t)rng.mti;
-        w[CHROMO_RNG_PHILOX_CTR] = (uint32_t)rng.ctr;
-        w[CHROMO_RNG_PHILOX_CTR + 1] = (uint32_t)(rng.ctr >> 32);
-        w[CHROMO_RNG_PHILOX_BUF] = rng.b0;
-        w[CHROMO_RNG_PHILOX_BUF + 1] = rng.b1;
-        w[CHROMO_RNG_PHILOX_BUF + 2] = rng.b2;
-        w[CHROMO_RNG_PHILOX_BUF + 3] = rng.b3;
-        w[CHROMO_RNG_PHILOX_IDX] = (uint32_t)rng.bi;
+        if (philox) {
+            w[CHROMO_RNG_PHILOX_CTR] = (uint32_t)ctr;
+            w[CHROMO_RNG_PHILOX_CTR + 1] = (uint32_t)(ctr >> 32);
+            for (int j = 0; j < 4; ++j) w[CHROMO_RNG_PHILOX_BUF + j] = b[j];
+            w[CHROMO_RNG_PHILOX_IDX] = (uint32_t)bi;
+        }
     }
 }
 

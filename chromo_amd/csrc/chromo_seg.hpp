@@ -11,11 +11,12 @@
 //     contribution index; a wave scan then numbers the first occurrences.  This
 //     is the reference's list of unique voxels in first-occurrence order
 //     (U = n_bins).
-//  2. Voxel u goes to lane u mod 64 (at most SEG_VB per lane).  The lane walks
-//     the positions from the voxel's first one on and folds every contribution
-//     that lands in it, in contribution order, with the 1e-18 rules of
-//     fields.pyx:1491-1520: the per-voxel delta density is the reference's left
-//     fold bit for bit.  The new densities stay in registers until the commit.
+//  2. Voxel u goes to lane u mod 64 (at most SEG_VB per lane).  The hash also
+//     keeps, per voxel, a 64-bit mask of the positions that touch it; the lane
+//     visits those positions in increasing order and folds their contributions,
+//     in contribution order, with the 1e-18 rules of fields.pyx:1491-1520: the
+//     per-voxel delta density is the reference's left fold bit for bit.  The new
+//     densities stay in registers until the commit.
 //  3. Energy terms, wave reduction (fixed order), Metropolis.
 //
 // A move with more than SEG_VB * 64 unique voxels is reported as an overflow;
@@ -39,7 +40,7 @@ constexpr int SEG_HASH_SHIFT = 22;            // 32 - log2(SEG_HASH)
 static_assert(SEG_POS <= 64, "one position per lane");
 
 struct SegLayout {
-    size_t c0, prec, hkey, hmin, eslot, uvox, ucrd, uaux, total;
+    size_t c0, pw, prec, hpos, umask, hkey, hmin, eslot, uvox, ucrd, uaux, total;
 };
 
 // Per-wavefront LDS block of the evaluator.
@@ -48,20 +49,24 @@ __host__ __device__ inline SegLayout seg_layout(int)
     SegLayout S;
     size_t o = 0;
     S.c0 = o;    o += sizeof(double) * SEG_E;          // corner weight / vol_bin per contribution
+    S.pw = o;    o += sizeof(double) * 4 * SEG_POS;    // lower-corner weights per position
     S.prec = o;  o += 16 * (size_t)SEG_POS;            // position records (int4)
+    S.hpos = o;  o += sizeof(uint64_t) * SEG_HASH;     // voxel -> positions touching it (bit mask)
+    S.umask = o; o += sizeof(uint64_t) * SEG_UMAX;     // the same per unique voxel
     S.hkey = o;  o += sizeof(int32_t) * SEG_HASH;      // voxel -> first contribution
     S.hmin = o;  o += sizeof(int32_t) * SEG_HASH;
     S.eslot = o; o += sizeof(int32_t) * SEG_E;         // hash slot per contribution
     S.uvox = o;  o += sizeof(int32_t) * SEG_UMAX;      // unique voxel index
-    S.ucrd = o;  o += 8 * (size_t)SEG_UMAX;            // its cell (int2: x | y << 16, z | first position << 16)
+    S.ucrd = o;  o += 8 * (size_t)SEG_UMAX;            // its cell (int2: x | y << 16, z)
     S.uaux = o;  o += sizeof(int32_t) * SEG_UMAX;      // hash slot; later a conflict-table slot
     S.total = (o + 255) / 256 * 256;
     return S;
 }
 
 struct SegLds {
-    double *c0;
+    double *c0, *pw;
     int4 *prec;
+    uint64_t *hpos, *umask;
     int32_t *hkey, *hmin, *eslot, *uvox, *uaux;
     int2 *ucrd;
 };
@@ -71,7 +76,10 @@ __device__ __forceinline__ SegLds seg_lds(char *base, int Bp1)
     const SegLayout L = seg_layout(Bp1);
     SegLds S;
     S.c0 = (double *)(base + L.c0);
+    S.pw = (double *)(base + L.pw);
     S.prec = (int4 *)(base + L.prec);
+    S.hpos = (uint64_t *)(base + L.hpos);
+    S.umask = (uint64_t *)(base + L.umask);
     S.hkey = (int32_t *)(base + L.hkey);
     S.hmin = (int32_t *)(base + L.hmin);
     S.eslot = (int32_t *)(base + L.eslot);
@@ -87,6 +95,7 @@ __device__ __forceinline__ void seg_clear(const SegLds &S)
     for (int i = lane_id(); i < SEG_HASH; i += WAVE) {
         S.hkey[i] = HKEY_EMPTY;
         S.hmin[i] = HMIN_EMPTY;
+        S.hpos[i] = 0;
     }
     wave_sync();
 }
@@ -174,7 +183,8 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
     constexpr int Bp1 = NB + 1;
     const int lane = lane_id();
     const int64_t N = R->N;
-    const double *Pp = R->pos;
+    const GLOBAL_AS double *Pp = gmem(R->pos);
+    const GLOBAL_AS int8_t *states = gmem(R->states);
     const int mv = P.move;
     const int64_t n = P.n, ind0 = P.ind0, indf = P.indf;
     const bool tangent = mv == CHROMO_TANGENT_ROTATION;
@@ -198,7 +208,7 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
         for (int c = 0; c < 9; ++c) o[c] = Pp[c * N + b];
 #pragma unroll
         for (int m = 0; m < NB; ++m) {
-            so[m] = R->states[(int64_t)m * N + b];
+            so[m] = states[(int64_t)m * N + b];
             sn[m] = so[m];
         }
         if (state_change) {
@@ -238,7 +248,7 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
         } else if (act) {
             int8_t mods[NB];
 #pragma unroll
-            for (int m = 0; m < NB; ++m) mods[m] = R->marks[(int64_t)m * N + b];
+            for (int m = 0; m < NB; ++m) mods[m] = gmem(R->marks)[(int64_t)m * N + b];
             e = bead_binding_dE<NB>(R, mods, so, sn);
         }
         for (int i = 0; i < n; ++i) dEp += readlane_f64(e, i);
@@ -290,7 +300,7 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
                 S.prec[p] = make_int4(x | (wrap1(x, nx) << 16), y | (wrap1(y, ny) << 16),
                                       z | (wrap1(z, nz) << 16), sp);
 #pragma unroll
-                for (int l = 0; l < 8; ++l) S.c0[p * 8 + l] = corner_weight(l, wt[0], wt[1], wt[2]) / vol_bin;
+                for (int j = 0; j < 3; ++j) S.pw[4 * p + j] = wt[j];
             }
         }
         wave_sync();
@@ -305,6 +315,9 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
             if (e < E) {
                 const int4 pr = S.prec[e >> 3];
                 const int l = e & 7;
+                // weight / access volume of this contribution (fields.pyx:1472-1488)
+                const double *wq = &S.pw[4 * (e >> 3)];
+                S.c0[e] = corner_weight(l, wq[0], wq[1], wq[2]) / vol_bin;
                 const int x = (l & 1) ? (pr.x >> 16) : (pr.x & 0xffff);
                 const int y = (l & 2) ? (pr.y >> 16) : (pr.y & 0xffff);
                 const int z = (l & 4) ? (pr.z >> 16) : (pr.z & 0xffff);
@@ -317,6 +330,7 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
                     h = (h + 1) & (SEG_HASH - 1);
                 }
                 atomicMin(&S.hmin[h], e);
+                atomicOr((unsigned long long *)&S.hpos[h], 1ull << (e >> 3));
                 S.eslot[e] = (int)h;
             }
         }
@@ -337,7 +351,8 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
                 const int y = (l & 2) ? (pr.y >> 16) : (pr.y & 0xffff);
                 const int z = (l & 4) ? (pr.z >> 16) : (pr.z & 0xffff);
                 S.uvox[q] = x + nx * (y + ny * z);
-                S.ucrd[q] = make_int2(x | (y << 16), z | ((e >> 3) << 16));
+                S.ucrd[q] = make_int2(x | (y << 16), z);
+                S.umask[q] = S.hpos[h];
                 S.uaux[q] = h;
             }
             total += cnt;
@@ -349,6 +364,7 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
             const int h = S.uaux[q];
             S.hkey[h] = HKEY_EMPTY;
             S.hmin[h] = HMIN_EMPTY;
+            S.hpos[h] = 0;
         }
         if (U > SEG_UMAX) {
             // too many voxels for the registers: the global-arena path evaluates it
@@ -356,6 +372,7 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
                 const int h = S.eslot[e];
                 S.hkey[h] = HKEY_EMPTY;
                 S.hmin[h] = HMIN_EMPTY;
+                S.hpos[h] = 0;
             }
             wave_sync();
             res.overflow = 1;
@@ -366,7 +383,7 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
 
         // 2. fold per voxel, voxel lane + 64 j on this lane
         const bool degen = nx == 1 || ny == 1 || nz == 1;
-        const double *rho_g = R->density;
+        const GLOBAL_AS double *rho_g = gmem(R->density);
         const int npos = (int)(2 * n);
         double S2[NB], SX[NB * NB], NS = 0;
 #pragma unroll
@@ -387,6 +404,7 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
                 acc[m] = 0;
             }
             int vx = -1, vy = -1, vz = -1;
+            uint64_t hits = 0;
             if (own) {
                 const int v = S.uvox[q];
 #pragma unroll
@@ -394,11 +412,14 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
                 const int2 c = S.ucrd[q];
                 vx = c.x & 0xffff;
                 vy = c.x >> 16;
-                vz = c.y & 0xffff;
+                vz = c.y;
+                hits = S.umask[q];
             }
-            // this batch's voxels first occur at or after its first voxel's position
-            const int p0 = S.ucrd[WAVE * j].y >> 16;
-            for (int p = p0; p < npos; ++p) {
+            // each lane visits the positions that touch its voxel, in order
+            while (wave_any(hits != 0)) {
+                if (hits == 0) continue;
+                const int p = __builtin_ctzll(hits);
+                hits &= hits - 1;
                 const int4 pr = S.prec[p];
                 const double pre = (p & 1) ? 1. : -1.;
                 const int x0 = pr.x & 0xffff, x1 = pr.x >> 16, y0 = pr.y & 0xffff, y1 = pr.y >> 16,
@@ -474,16 +495,16 @@ __device__ __forceinline__ void seg_commit(const chromo_replica *R, int move, in
     const int64_t N = R->N;
     if (lane < n) {
         if (move == CHROMO_CHANGE_BINDING_STATE) {
-            R->states[(int64_t)binder * N + K.b] = (int8_t)mynew;
+            gmem(R->states)[(int64_t)binder * N + K.b] = (int8_t)mynew;
         } else {
             const int c0 = (move == CHROMO_TANGENT_ROTATION) ? 3 : 0;
             const int c1 = (move == CHROMO_SLIDE) ? 3 : 9;
 #pragma unroll
             for (int c = 0; c < 9; ++c)
-                if (c >= c0 && c < c1) R->pos[c * N + K.b] = K.w[c];
+                if (c >= c0 && c < c1) gmem(R->pos)[c * N + K.b] = K.w[c];
         }
     }
-    double *rho = R->density;
+    GLOBAL_AS double *rho = gmem(R->density);
 #pragma unroll
     for (int j = 0; j < SEG_VB; ++j) {
         const int q = lane + WAVE * j;

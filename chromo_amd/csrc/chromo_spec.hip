@@ -20,7 +20,7 @@ namespace {
 // wavefronts: the register budget is sized for that.
 template <int W, int NB>
 __global__ void __launch_bounds__(WAVE * W) __attribute__((amdgpu_waves_per_eu(W / 2 > 1 ? W / 2 : 1)))
-k_mc_spec(const chromo_replica *reps, int n_reps, int64_t steps)
+k_mc_spec(const chromo_replica *__restrict__ reps, int n_reps, int64_t steps)
 {
     const int r = blockIdx.x;
     if (r >= n_reps) return;

@@ -109,17 +109,21 @@ struct SpecLds {
 __device__ __forceinline__ void rng_save(const WaveRng &g, uint32_t *ck)
 {
     const int lane = lane_id();
+    uint64_t ctr = 0;
+    int bi = 4;
+    uint32_t b[4] = {0, 0, 0, 0};
+    if (g.mode == CHROMO_RNG_PHILOX) philox_get(g, ctr, bi, b);
     if (lane < 31) ck[lane] = g.g;
     if (lane == 0) {
         ck[31] = (uint32_t)g.gf;
         ck[32] = (uint32_t)g.gr;
-        ck[33] = (uint32_t)g.ctr;
-        ck[34] = (uint32_t)(g.ctr >> 32);
-        ck[35] = g.b0;
-        ck[36] = g.b1;
-        ck[37] = g.b2;
-        ck[38] = g.b3;
-        ck[39] = (uint32_t)g.bi;
+        ck[33] = (uint32_t)ctr;
+        ck[34] = (uint32_t)(ctr >> 32);
+        ck[35] = b[0];
+        ck[36] = b[1];
+        ck[37] = b[2];
+        ck[38] = b[3];
+        ck[39] = (uint32_t)bi;
     }
     wave_sync();
 }
@@ -133,12 +137,7 @@ __device__ __forceinline__ void rng_restore(WaveRng &g, const uint32_t *ck)
     g.g = lane < 31 ? ck[lane] : 0u;
     g.gf = uni_i32((int)ck[31]);
     g.gr = uni_i32((int)ck[32]);
-    g.ctr = uni_u64((uint64_t)ck[33] | ((uint64_t)ck[34] << 32));
-    g.b0 = uni_u32(ck[35]);
-    g.b1 = uni_u32(ck[36]);
-    g.b2 = uni_u32(ck[37]);
-    g.b3 = uni_u32(ck[38]);
-    g.bi = uni_i32((int)ck[39]);
+    if (g.mode == CHROMO_RNG_PHILOX) philox_set(g, (uint64_t)ck[33] | ((uint64_t)ck[34] << 32), (int)ck[39]);
 }
 
 // Draws one proposal into slot S (wavefront 0), in the draw order of
@@ -217,7 +216,7 @@ __device__ __forceinline__ int gen_slot(const chromo_replica *R, WaveRng &rng, c
                         break;
                     }
                     int64_t c = draw_mod(rnd31(rng), N);
-                    if (!__any(lane < i && mine == c)) {
+                    if (!wave_any(lane < i && mine == c)) {
                         if (lane == i) mine = c;
                         break;
                     }
@@ -287,11 +286,14 @@ __device__ __forceinline__ void seq_step(const chromo_replica *R, SeqIO *io, uin
 {
     WaveRng rng;
     rng.mode = uni_i32(R->rng_mode);
+    rng.k0 = uni_u32(R->rng[CHROMO_RNG_PHILOX_KEY]);
+    rng.k1 = uni_u32(R->rng[CHROMO_RNG_PHILOX_KEY + 1]);
+    rng.pbase = PHILOX_NONE;
+    rng.pidx = 0;
+    rng.p0 = rng.p1 = rng.p2 = rng.p3 = 0;
     rng_restore(rng, io->ck);
     rng.mt = mt;
     rng.mti = uni_i32(io->mti);
-    rng.k0 = uni_u32(R->rng[CHROMO_RNG_PHILOX_KEY]);
-    rng.k1 = uni_u32(R->rng[CHROMO_RNG_PHILOX_KEY + 1]);
     chromo_ctrl C = io->C;
     MoveOut o;
     mc_step_dev<NB>(R, rng, C, o, G);
@@ -454,7 +456,7 @@ __device__ __forceinline__ bool eval_slot(const chromo_replica *R, const SpecSlo
             }
         }
     }
-    full = __any(full);
+    full = wave_any(full);
     if (lane == 0) {
         res.dEp = sr.dEp;
         res.dEf = sr.dEf;
