@@ -166,8 +166,22 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each sweep directly instead of replaying a captured HIP graph")
     ap.add_argument("--waves", default="auto",
-                    help="wavefronts per replica (1, 2, 4) or 'auto' to time each and keep the fastest")
+                    help="wavefronts per replica (1 or 2) or 'auto' to time each and keep the fastest")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 256 MiB device to device 5 times (calibrates rocprofv3's "
+                         "FETCH_SIZE / WRITE_SIZE counters against a known byte count)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = 256 * 1024 * 1024 // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        c = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            c.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copied_bytes_per_launch": n * 8, "launches": 5}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: replicas are independent chains; multi-GPU runs "
               "would shard replicas with no exchange and are not part of this build.",
@@ -191,7 +205,8 @@ def main():
 
     achieved = b["bytes"] / b["secs"] / 1e9
     traffic = None
-    prof = ROOT / "profiles" / "traffic_k_mc_run.json"
+    kname = "k_mc_run" if b["waves"] == 1 else f"k_mc_spec_w{b['waves']}"
+    prof = ROOT / "profiles" / f"traffic_{kname}.json"
     if prof.exists():
         try:
             traffic = json.loads(prof.read_text()).get("hbm_bytes_per_launch")
