@@ -384,11 +384,50 @@ __device__ __forceinline__ int32_t rnd31(WaveRng &R)
 
 __device__ __forceinline__ double rnd_unit(WaveRng &R) { return (double)rnd31(R) / RMAX; }
 
+// Per lane, the Philox word t draws ahead of the next one (t = 0 is the next
+// draw); valid while pidx + t < 4 * WAVE.  Advances nothing.
+__device__ __forceinline__ uint32_t philox_peek(const WaveRng &R, int t)
+{
+    const int idx = R.pidx + t;
+    const int addr = (idx >> 2) << 2;   // byte address of the source lane
+    const int w = idx & 3;
+    const uint32_t a = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)R.p0);
+    const uint32_t b = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)R.p1);
+    const uint32_t c = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)R.p2);
+    const uint32_t d = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)R.p3);
+    return w == 0 ? a : (w == 1 ? b : (w == 2 ? c : d));
+}
+
 // C's x % m for a draw x in [0, 2^31) and m >= 1, in 32-bit arithmetic.
 __device__ __forceinline__ int64_t draw_mod(int32_t x, int64_t m)
 {
     if (m > 0x7fffffff) return x;
     return (int64_t)((uint32_t)x % (uint32_t)m);
+}
+
+// The draws of a tangent rotation after its angle and count
+// (move_funcs.pyx:470-582): k distinct beads, each rand() % N with a repeat drawn
+// again, then two unit-sphere draws per bead.  When no bead repeats, these are
+// the next 3k draws; in the Philox mode, with all of them in the current batch,
+// lane i < k reads bead i and its two draws from the batch directly.  Returns
+// false, having drawn nothing, when that does not apply: the caller then draws
+// them one at a time.
+__device__ __forceinline__ bool tangent_draws_at_once(WaveRng &R, int64_t k, int64_t N, int64_t &pick,
+                                                      int32_t &d1, int32_t &d2)
+{
+    if (R.mode != CHROMO_RNG_PHILOX || k > WAVE || R.pidx + 3 * k > 4 * WAVE) return false;
+    const int lane = lane_id();
+    const bool act = lane < k;
+    const int64_t c = draw_mod((int32_t)(philox_peek(R, act ? lane : 0) >> 1), N);
+    bool rep = false;
+    for (int j = 0; j + 1 < k; ++j) rep |= act && lane > j && c == readlane_i64(c, j);
+    if (wave_any(rep)) return false;
+    const int t = (int)k + 2 * lane;
+    d1 = (int32_t)(philox_peek(R, act ? t : 0) >> 1);
+    d2 = (int32_t)(philox_peek(R, act ? t + 1 : 0) >> 1);
+    pick = act ? c : 0;
+    R.pidx += (int)(3 * k);
+    return true;
 }
 
 // numpy legacy RandomState.randint(0, high): masked rejection over 32-bit draws

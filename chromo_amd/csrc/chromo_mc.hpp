@@ -709,6 +709,24 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
             goto fail;
         }
         n = k;
+        const double origin[3] = {0., 0., 0.};
+        int64_t fpick = 0;
+        int32_t fd1 = 0, fd2 = 0;
+        if (tangent_draws_at_once(rng, k, N, fpick, fd1, fd2)) {
+            // every lane its own bead and rotation
+            if (lane < k) {
+                double dir[3], T[12];
+                sphere_from(fd1, fd2, dir);
+                axis_rotation(dir, origin, ang, T);
+                P.mypick = fpick;
+                for (int j = 0; j < 3; ++j)
+                    for (int q = 0; q < 3; ++q) P.myT[3 * j + q] = T[4 * j + q];
+            }
+            ind0 = 0;
+            indf = 0;
+            goto picked;
+        }
+        {
         // beads past the first 64 keep their picks and rotations in the global arena
         Arena A = make_arena((char *)R->scratch, global_layout(R->bead_cap, R->seg_cap, NB + 1));
         for (int64_t i = 0; i < k && !err; ++i) {
@@ -732,7 +750,6 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
             }
         }
         if (err) goto fail;
-        const double origin[3] = {0., 0., 0.};
         for (int64_t i = 0; i < k; ++i) {
             double dir[3], T[12];
             sample_unit_sphere(rng, dir);
@@ -750,6 +767,8 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
         wave_sync_global();
         ind0 = 0;
         indf = 0;
+        }
+    picked:;
     } else {
         // change_binding_state, move_funcs.pyx:717-820
         if (R->B < 1) {

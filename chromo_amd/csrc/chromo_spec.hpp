@@ -206,7 +206,15 @@ __device__ __forceinline__ int gen_slot(const chromo_replica *R, WaveRng &rng, c
         int64_t k = draw_mod(rnd31(rng), amp_bead) + 1;
         n = k;
         seq = (k > N) || (k > SPEC_NMAX);
-        if (!seq) {
+        int64_t fpick = 0;
+        int32_t fd1 = 0, fd2 = 0;
+        if (!seq && tangent_draws_at_once(rng, k, N, fpick, fd1, fd2)) {
+            if (lane < k) {
+                S.picks[lane] = fpick;
+                S.ps1[lane] = fd1;
+                S.ps2[lane] = fd2;
+            }
+        } else if (!seq) {
             int64_t mine = 0;
             for (int64_t i = 0; i < k && !seq; ++i) {
                 for (int it = 0;; ++it) {
