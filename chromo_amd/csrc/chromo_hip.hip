@@ -38,7 +38,7 @@ __global__ void __launch_bounds__(64) k_mc_run(const chromo_replica *__restrict_
     const int Bp1 = R->B + 1;
     const SegLds G = seg_lds(lds_dyn, Bp1);
     seg_clear(G);
-    uint32_t *mt_lds = (uint32_t *)(lds_dyn + seg_layout(Bp1).total);
+    uint32_t *mt_lds = (uint32_t *)(lds_dyn + seg_layout<SEG_WIDE>(Bp1).total);
     const bool need_mt = any_binding(R);
     WaveRng rng;
     rng_load(R, rng, mt_lds, need_mt);
@@ -51,10 +51,10 @@ __global__ void __launch_bounds__(64) k_mc_run(const chromo_replica *__restrict_
     __syncthreads();
     int err = 0;
     switch (R->B) {
-    case 1: run_steps<1>(R, rng, K, sctrl, nc, steps, err, G); break;
-    case 2: run_steps<2>(R, rng, K, sctrl, nc, steps, err, G); break;
-    case 3: run_steps<3>(R, rng, K, sctrl, nc, steps, err, G); break;
-    case 4: run_steps<4>(R, rng, K, sctrl, nc, steps, err, G); break;
+    case 1: run_steps<1>(R, rng, K, sctrl, nc, steps, err, G, lds_dyn); break;
+    case 2: run_steps<2>(R, rng, K, sctrl, nc, steps, err, G, lds_dyn); break;
+    case 3: run_steps<3>(R, rng, K, sctrl, nc, steps, err, G, lds_dyn); break;
+    case 4: run_steps<4>(R, rng, K, sctrl, nc, steps, err, G, lds_dyn); break;
     default: err = ERR_NBINDERS; break;
     }
     rng_store(R, rng, need_mt);
@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(64) k_mc_step(const chromo_replica *__restrict
     const int Bp1 = R->B + 1;
     const SegLds G = seg_lds(lds_dyn, Bp1);
     seg_clear(G);
-    uint32_t *mt_lds = (uint32_t *)(lds_dyn + seg_layout(Bp1).total);
+    uint32_t *mt_lds = (uint32_t *)(lds_dyn + seg_layout<SEG_WIDE>(Bp1).total);
     chromo_ctrl C = R->ctrl[ci];
     const bool need_mt = (C.move == CHROMO_CHANGE_BINDING_STATE);
     WaveRng rng;
@@ -85,10 +85,10 @@ __global__ void __launch_bounds__(64) k_mc_step(const chromo_replica *__restrict
     counters_load(R, K);
     MoveOut o;
     switch (R->B) {
-    case 1: mc_step_dev<1>(R, rng, C, o, G); break;
-    case 2: mc_step_dev<2>(R, rng, C, o, G); break;
-    case 3: mc_step_dev<3>(R, rng, C, o, G); break;
-    case 4: mc_step_dev<4>(R, rng, C, o, G); break;
+    case 1: mc_step_dev<1>(R, rng, C, o, G, lds_dyn, 1); break;
+    case 2: mc_step_dev<2>(R, rng, C, o, G, lds_dyn, 1); break;
+    case 3: mc_step_dev<3>(R, rng, C, o, G, lds_dyn, 1); break;
+    case 4: mc_step_dev<4>(R, rng, C, o, G, lds_dyn, 1); break;
     default: o.err = ERR_NBINDERS; o.n = 0; break;
     }
     if (!o.err && o.n > 0) record(R, C, o, K);

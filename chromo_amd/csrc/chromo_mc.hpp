@@ -155,9 +155,10 @@ inline size_t lds_bytes_for(int Bp1)
     return L.total + MT_LDS_BYTES;
 }
 
-// Dynamic LDS of the one-wavefront Monte Carlo kernels: one evaluator block, then
-// the MT19937 state.
-inline size_t mc_lds_bytes(int Bp1) { return seg_layout(Bp1).total + MT_LDS_BYTES; }
+// Dynamic LDS of the one-wavefront Monte Carlo kernels: one evaluator block for
+// moves of up to SEG_WIDE beads (its start doubles as the SEG_MAX block), then the
+// MT19937 state.
+inline size_t mc_lds_bytes(int Bp1) { return seg_layout<SEG_WIDE>(Bp1).total + MT_LDS_BYTES; }
 
 extern __shared__ char lds_dyn[];
 
@@ -597,9 +598,12 @@ __device__ __forceinline__ void load_pos3(const double *Pp_, int64_t N, int64_t 
     v[2] = Pp[2 * N + b];
 }
 
+// G: the caller's SEG_MAX evaluator block.  wide: the start of an LDS region that
+// holds a SEG_WIDE block (nullptr if none) and overlaps nstd SEG_MAX blocks, whose
+// hashes are emptied again after it is used.
 template <int NB>
 __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rng, chromo_ctrl &C, MoveOut &out,
-                                            const SegLds &G)
+                                            const SegLds &G, char *wide, int nstd)
 {
     const int lane = lane_id();
     const int64_t N = R->N;
@@ -828,13 +832,27 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
             if (!sr.overflow) {
                 if (sr.accepted) seg_commit<NB>(R, P.move, n, P.binder, P.mynew, K, G, sr.U);
                 wave_sync_global();
-                out.dEp = sr.dEp;
-                out.dEf = sr.dEf;
-                out.u = u;
-                out.expdE = sr.expdE;
-                out.accepted = sr.accepted;
-                out.nbins = sr.U;
             }
+        } else if (n <= SEG_WIDE && wide != nullptr) {
+            const SegLds GW = seg_lds<SEG_WIDE>(wide, NB + 1);
+            seg_clear<SEG_WIDE>(GW);
+            SegKeep<NB> K;
+            seg_eval<NB, SEG_WIDE>(R, P, u, GW, sr, K);
+            if (!sr.overflow) {
+                if (sr.accepted) seg_commit<NB>(R, P.move, n, P.binder, P.mynew, K, GW, sr.U);
+                wave_sync_global();
+            }
+            // the wide block covered the SEG_MAX blocks: their hashes must be empty again
+            for (int b = 0; b < nstd; ++b)
+                seg_clear<SEG_MAX>(seg_lds<SEG_MAX>(wide + (size_t)b * seg_layout<SEG_MAX>(NB + 1).total, NB + 1));
+        }
+        if (!sr.overflow) {
+            out.dEp = sr.dEp;
+            out.dEf = sr.dEf;
+            out.u = u;
+            out.expdE = sr.expdE;
+            out.accepted = sr.accepted;
+            out.nbins = sr.U;
         }
         // larger moves, and the rare small one with too many voxels for seg_eval
         if (sr.overflow) {
@@ -1041,7 +1059,7 @@ __device__ __forceinline__ bool any_binding(const chromo_replica *R)
 
 template <int NB>
 __device__ __forceinline__ void run_steps(const chromo_replica *R, WaveRng &rng, Counters &K, chromo_ctrl *sctrl,
-                          int nc, int64_t steps, int &err, const SegLds &G)
+                          int nc, int64_t steps, int &err, const SegLds &G, char *wide)
 {
     for (int64_t s = 0; s < steps && !err; ++s) {
         for (int ci = 0; ci < nc && !err; ++ci) {
@@ -1049,7 +1067,7 @@ __device__ __forceinline__ void run_steps(const chromo_replica *R, WaveRng &rng,
             if (c.on == 1) {
                 for (int j = 0; j < c.num_per_cycle; ++j) {
                     MoveOut o;
-                    mc_step_dev<NB>(R, rng, c, o, G);
+                    mc_step_dev<NB>(R, rng, c, o, G, wide, 1);
                     if (o.err) {
                         err = o.err;
                         break;

@@ -21,6 +21,9 @@
 //
 // A move with more than SEG_VB * 64 unique voxels is reported as an overflow;
 // the caller evaluates it through the global-arena path instead.
+//
+// The code is built for two capacities: SMAX = SEG_MAX (32 beads, the block every
+// wavefront owns) and SMAX = SEG_WIDE (64 beads, a block that spans two of those).
 #pragma once
 
 #include "chromo_device.hpp"
@@ -29,36 +32,46 @@ namespace {
 
 using namespace chromo;
 
-constexpr int SEG_MAX = 32;                   // beads per move
-constexpr int SEG_POS = 2 * SEG_MAX;          // positions (<= one per lane)
-constexpr int SEG_E = 8 * SEG_POS;            // contributions
-constexpr int SEG_ROUNDS = SEG_E / 64;        // contribution rounds of 64 lanes
+constexpr int SEG_MAX = 32;                   // beads per move, per-wavefront block
+constexpr int SEG_WIDE = 64;                  // beads per move, two-block block
 constexpr int SEG_VB = 4;                     // unique voxels per lane
 constexpr int SEG_UMAX = SEG_VB * 64;         // unique voxels one evaluation holds
-constexpr int SEG_HASH = 1024;                // >= 2 * SEG_E
+constexpr int SEG_HASH = 1024;                // hash slots
 constexpr int SEG_HASH_SHIFT = 22;            // 32 - log2(SEG_HASH)
-static_assert(SEG_POS <= 64, "one position per lane");
+
+// Sizes for a capacity of SMAX beads (one bead per lane, two positions per bead).
+template <int SMAX>
+struct SegCap {
+    static_assert(SMAX <= 64, "one bead per lane");
+    static constexpr int POS = 2 * SMAX;           // positions
+    static constexpr int E = 8 * POS;              // contributions
+    static constexpr int ROUNDS = E / 64;          // contribution rounds of 64 lanes
+    static constexpr int MW = (POS + 63) / 64;     // 64-bit words of a position mask
+    static_assert(E <= SEG_HASH, "at most one hash slot per contribution");
+};
 
 struct SegLayout {
     size_t c0, pw, prec, hpos, umask, hkey, hmin, eslot, uvox, ucrd, uaux, total;
 };
 
-// Per-wavefront LDS block of the evaluator.
+// LDS block of the evaluator for moves of up to SMAX beads.
+template <int SMAX = SEG_MAX>
 __host__ __device__ inline SegLayout seg_layout(int)
 {
+    using C = SegCap<SMAX>;
     SegLayout S;
     size_t o = 0;
-    S.c0 = o;    o += sizeof(double) * SEG_E;          // corner weight / vol_bin per contribution
-    S.pw = o;    o += sizeof(double) * 4 * SEG_POS;    // lower-corner weights per position
-    S.prec = o;  o += 16 * (size_t)SEG_POS;            // position records (int4)
-    S.hpos = o;  o += sizeof(uint64_t) * SEG_HASH;     // voxel -> positions touching it (bit mask)
-    S.umask = o; o += sizeof(uint64_t) * SEG_UMAX;     // the same per unique voxel
-    S.hkey = o;  o += sizeof(int32_t) * SEG_HASH;      // voxel -> first contribution
+    S.c0 = o;    o += sizeof(double) * C::E;                   // corner weight / vol_bin per contribution
+    S.pw = o;    o += sizeof(double) * 4 * C::POS;             // lower-corner weights per position
+    S.prec = o;  o += 16 * (size_t)C::POS;                     // position records (int4)
+    S.hpos = o;  o += sizeof(uint64_t) * C::MW * SEG_HASH;     // voxel -> positions touching it (bit mask)
+    S.umask = o; o += sizeof(uint64_t) * C::MW * SEG_UMAX;     // the same per unique voxel
+    S.hkey = o;  o += sizeof(int32_t) * SEG_HASH;              // voxel -> first contribution
     S.hmin = o;  o += sizeof(int32_t) * SEG_HASH;
-    S.eslot = o; o += sizeof(int32_t) * SEG_E;         // hash slot per contribution
-    S.uvox = o;  o += sizeof(int32_t) * SEG_UMAX;      // unique voxel index
-    S.ucrd = o;  o += 8 * (size_t)SEG_UMAX;            // its cell (int2: x | y << 16, z)
-    S.uaux = o;  o += sizeof(int32_t) * SEG_UMAX;      // hash slot; later a conflict-table slot
+    S.eslot = o; o += sizeof(int32_t) * C::E;                  // hash slot per contribution
+    S.uvox = o;  o += sizeof(int32_t) * SEG_UMAX;              // unique voxel index
+    S.ucrd = o;  o += 8 * (size_t)SEG_UMAX;                    // its cell (int2: x | y << 16, z)
+    S.uaux = o;  o += sizeof(int32_t) * SEG_UMAX;              // hash slot; later a conflict-table slot
     S.total = (o + 255) / 256 * 256;
     return S;
 }
@@ -71,9 +84,10 @@ struct SegLds {
     int2 *ucrd;
 };
 
+template <int SMAX = SEG_MAX>
 __device__ __forceinline__ SegLds seg_lds(char *base, int Bp1)
 {
-    const SegLayout L = seg_layout(Bp1);
+    const SegLayout L = seg_layout<SMAX>(Bp1);
     SegLds S;
     S.c0 = (double *)(base + L.c0);
     S.pw = (double *)(base + L.pw);
@@ -90,12 +104,14 @@ __device__ __forceinline__ SegLds seg_lds(char *base, int Bp1)
 }
 
 // Empties the evaluator's hash (once per launch, by its wavefront).
+template <int SMAX = SEG_MAX>
 __device__ __forceinline__ void seg_clear(const SegLds &S)
 {
     for (int i = lane_id(); i < SEG_HASH; i += WAVE) {
         S.hkey[i] = HKEY_EMPTY;
         S.hmin[i] = HMIN_EMPTY;
-        S.hpos[i] = 0;
+#pragma unroll
+        for (int w = 0; w < SegCap<SMAX>::MW; ++w) S.hpos[i * SegCap<SMAX>::MW + w] = 0;
     }
     wave_sync();
 }
@@ -171,15 +187,17 @@ __device__ __forceinline__ void seg_transform(const Proposal &P, const double o[
 
 __device__ __forceinline__ int wrap1(int c, int nn) { return c + 1 == nn ? 0 : c + 1; }
 
-// Evaluates proposal P (1 <= n <= SEG_MAX) against the committed state: polymer
+// Evaluates proposal P (1 <= n <= SMAX) against the committed state: polymer
 // dE (SSWLC.compute_dE, polymers.pyx:1024-1082), field dE
 // (UniformDensityField.compute_dE, fields.pyx:1149-1233) and the Metropolis test
 // with the uniform draw u (mc_sim.pyx:152-182).  Writes nothing outside S and K.
 // Called by one whole wavefront; S's hash is empty on entry and on return.
-template <int NB>
+template <int NB, int SMAX = SEG_MAX>
 __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal &P, double u, const SegLds &S,
                                          SegRes &res, SegKeep<NB> &K)
 {
+    using C = SegCap<SMAX>;
+    constexpr int MW = C::MW;
     constexpr int Bp1 = NB + 1;
     const int lane = lane_id();
     const int64_t N = R->N;
@@ -309,7 +327,7 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
         const int E = (int)(16 * n);
         const int rounds = (E + WAVE - 1) / WAVE;
 #pragma unroll
-        for (int r = 0; r < SEG_ROUNDS; ++r) {
+        for (int r = 0; r < C::ROUNDS; ++r) {
             if (r >= rounds) break;
             const int e = lane + WAVE * r;
             if (e < E) {
@@ -323,21 +341,22 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
                 const int z = (l & 4) ? (pr.z >> 16) : (pr.z & 0xffff);
                 const int v = x + nx * (y + ny * z);
                 uint32_t h = ((uint32_t)v * 2654435761u) >> SEG_HASH_SHIFT;
-                // at most SEG_E keys in 2 * SEG_E slots: the probe ends
+                // fewer distinct keys than slots before each insertion: the probe ends
                 for (int probes = 0; probes < SEG_HASH; ++probes) {
                     const int old = atomicCAS(&S.hkey[h], HKEY_EMPTY, v);
                     if (old == HKEY_EMPTY || old == v) break;
                     h = (h + 1) & (SEG_HASH - 1);
                 }
                 atomicMin(&S.hmin[h], e);
-                atomicOr((unsigned long long *)&S.hpos[h], 1ull << (e >> 3));
+                const int p = e >> 3;
+                atomicOr((unsigned long long *)&S.hpos[h * MW + (p >> 6)], 1ull << (p & 63));
                 S.eslot[e] = (int)h;
             }
         }
         wave_sync();
         int total = 0;
 #pragma unroll
-        for (int r = 0; r < SEG_ROUNDS; ++r) {
+        for (int r = 0; r < C::ROUNDS; ++r) {
             if (r >= rounds) break;
             const int e = lane + WAVE * r;
             const int h = e < E ? S.eslot[e] : 0;
@@ -352,7 +371,8 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
                 const int z = (l & 4) ? (pr.z >> 16) : (pr.z & 0xffff);
                 S.uvox[q] = x + nx * (y + ny * z);
                 S.ucrd[q] = make_int2(x | (y << 16), z);
-                S.umask[q] = S.hpos[h];
+#pragma unroll
+                for (int w = 0; w < MW; ++w) S.umask[q * MW + w] = S.hpos[h * MW + w];
                 S.uaux[q] = h;
             }
             total += cnt;
@@ -364,7 +384,8 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
             const int h = S.uaux[q];
             S.hkey[h] = HKEY_EMPTY;
             S.hmin[h] = HMIN_EMPTY;
-            S.hpos[h] = 0;
+#pragma unroll
+            for (int w = 0; w < MW; ++w) S.hpos[h * MW + w] = 0;
         }
         if (U > SEG_UMAX) {
             // too many voxels for the registers: the global-arena path evaluates it
@@ -372,7 +393,8 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
                 const int h = S.eslot[e];
                 S.hkey[h] = HKEY_EMPTY;
                 S.hmin[h] = HMIN_EMPTY;
-                S.hpos[h] = 0;
+#pragma unroll
+                for (int w = 0; w < MW; ++w) S.hpos[h * MW + w] = 0;
             }
             wave_sync();
             res.overflow = 1;
@@ -404,7 +426,9 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
                 acc[m] = 0;
             }
             int vx = -1, vy = -1, vz = -1;
-            uint64_t hits = 0;
+            uint64_t hits[MW];
+#pragma unroll
+            for (int w = 0; w < MW; ++w) hits[w] = 0;
             if (own) {
                 const int v = S.uvox[q];
 #pragma unroll
@@ -413,13 +437,23 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
                 vx = c.x & 0xffff;
                 vy = c.x >> 16;
                 vz = c.y;
-                hits = S.umask[q];
+#pragma unroll
+                for (int w = 0; w < MW; ++w) hits[w] = S.umask[q * MW + w];
             }
             // each lane visits the positions that touch its voxel, in order
-            while (wave_any(hits != 0)) {
-                if (hits == 0) continue;
-                const int p = __builtin_ctzll(hits);
-                hits &= hits - 1;
+            for (;;) {
+                bool more = false;
+#pragma unroll
+                for (int w = 0; w < MW; ++w) more |= hits[w] != 0;
+                if (!wave_any(more)) break;
+                if (!more) continue;
+                int p = 0;
+#pragma unroll
+                for (int w = MW - 1; w >= 0; --w)
+                    if (hits[w] != 0) p = 64 * w + __builtin_ctzll(hits[w]);
+#pragma unroll
+                for (int w = 0; w < MW; ++w)
+                    if (w == (p >> 6)) hits[w] &= hits[w] - 1;
                 const int4 pr = S.prec[p];
                 const double pre = (p & 1) ? 1. : -1.;
                 const int x0 = pr.x & 0xffff, x1 = pr.x >> 16, y0 = pr.y & 0xffff, y1 = pr.y >> 16,

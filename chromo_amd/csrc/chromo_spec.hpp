@@ -105,6 +105,7 @@ struct SpecLds {
     uint32_t *mt;
     SegLds mine;    // this wavefront's evaluator block
     SegLds first;   // wavefront 0's (the one-at-a-time path)
+    char *wide;     // the W blocks as one SEG_WIDE block, or nullptr if they are too small
 };
 __device__ __forceinline__ void rng_save(const WaveRng &g, uint32_t *ck)
 {
@@ -290,7 +291,8 @@ __device__ __forceinline__ int gen_slot(const chromo_replica *R, WaveRng &rng, c
 // callee save most of the register file on every call and the caller spill
 // around it.
 template <int NB, int W>
-__device__ __forceinline__ void seq_step(const chromo_replica *R, SeqIO *io, uint32_t *mt, const SegLds &G)
+__device__ __forceinline__ void seq_step(const chromo_replica *R, SeqIO *io, uint32_t *mt, const SegLds &G,
+                                         char *wide)
 {
     WaveRng rng;
     rng.mode = uni_i32(R->rng_mode);
@@ -304,7 +306,7 @@ __device__ __forceinline__ void seq_step(const chromo_replica *R, SeqIO *io, uin
     rng.mti = uni_i32(io->mti);
     chromo_ctrl C = io->C;
     MoveOut o;
-    mc_step_dev<NB>(R, rng, C, o, G);
+    mc_step_dev<NB>(R, rng, C, o, G, wide, W);
     rng_save(rng, io->ck);
     if (lane_id() == 0) {
         io->mti = rng.mti;
@@ -672,7 +674,9 @@ __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng
                     io->C = C;
                 }
                 wave_sync();
-                seq_step<NB, W>(R, io, L.mt, L.first);
+                // the other wavefronts wait at the barrier: wavefront 0 may use all
+                // W evaluator blocks as one wide block
+                seq_step<NB, W>(R, io, L.mt, L.first, L.wide);
                 rng_restore(rng, io->ck);
                 rng.mti = uni_i32(io->mti);
                 if (lane == 0) C = io->C;
@@ -739,6 +743,7 @@ __device__ __forceinline__ void spec_run(const chromo_replica *R, int64_t steps)
     SpecLds L;
     L.mine = seg_lds(lds_dyn + SL.seg + SL.seg_bytes * wv, Bp1);
     L.first = seg_lds(lds_dyn + SL.seg, Bp1);
+    L.wide = SL.seg_bytes * W >= seg_layout<SEG_WIDE>(Bp1).total ? lds_dyn + SL.seg : nullptr;
     seg_clear(L.mine);
     L.slots[0] = (SpecSlot *)(lds_dyn + SL.slots);
     L.slots[1] = L.slots[0] + W;
