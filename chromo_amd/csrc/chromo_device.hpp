@@ -505,15 +505,28 @@ __device__ __forceinline__ double rowdot(const double *Trow, double x0, double x
 }
 
 // Cython float modulo with cdivision=False (fmod + sign fix).
+// fmod(a, b) for b > 0 and |a / b| < 2^40, without branches.  The remainder
+// a - q b, q the truncated quotient, is representable, so one fma gives it exactly
+// once q is right; q = trunc(a / b) from the rounded division is off by at most one
+// when the quotient is near an integer, which the sign / range of a first fma
+// result shows.  Equal to fmod bit for bit, the sign of a zero included.
+__device__ __forceinline__ double fmod_exact(double a, double b)
+{
+    double q = trunc(a / b);
+    double r = fma(-q, b, a);
+    const double dq = (a >= 0) ? ((r < 0) ? -1.0 : (r >= b ? 1.0 : 0.0))
+                               : ((r > 0) ? 1.0 : (r <= -b ? -1.0 : 0.0));
+    q += dq;
+    r = fma(-q, b, a);
+    return r == 0 ? copysign(0.0, a) : r;
+}
+
 // Python's float a % b for b > 0: fmod (exact), then b added to a nonzero result of
-// the other sign.  For |a| < 2b fmod's exact result is a, a - b or a + b, and
-// those differences are exact (Sterbenz), so no library fmod is needed there.
+// the other sign.
 __device__ __forceinline__ double pymod(double a, double b)
 {
     double r;
-    if (b > 0 && fabs(a) < b) r = a;
-    else if (b > 0 && a >= b && a < 2 * b) r = a - b;
-    else if (b > 0 && a <= -b && a > -2 * b) r = copysign(a + b, a);   // fmod keeps a's sign, also for 0
+    if (b > 0 && fabs(a) < 0x1p40 * b) r = fmod_exact(a, b);
     else r = fmod(a, b);
     r += (double)((r != 0) & ((r < 0) ^ (b < 0))) * b;
     return r;

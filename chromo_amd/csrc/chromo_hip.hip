@@ -29,7 +29,10 @@ __global__ void __launch_bounds__(64) k_mc_run(const chromo_replica *__restrict_
 {
     const int r = blockIdx.x;
     if (r >= n_reps) return;
-    const chromo_replica *R = &reps[r];
+    __shared__ chromo_replica Rs;
+    load_record(&Rs, &reps[r]);
+    wave_sync();
+    const chromo_replica *R = &Rs;
     if (R->counters[CHROMO_CNT_ERROR] != 0) return;
     if (R->B < 1 || R->B + 1 > max_bp1) {
         counters_error(R, ERR_NBINDERS);
@@ -67,7 +70,10 @@ __global__ void __launch_bounds__(64) k_mc_run(const chromo_replica *__restrict_
 __global__ void __launch_bounds__(64) k_mc_step(const chromo_replica *__restrict__ reps, int rep, int ci,
                                                 int max_bp1)
 {
-    const chromo_replica *R = &reps[rep];
+    __shared__ chromo_replica Rs;
+    load_record(&Rs, &reps[rep]);
+    wave_sync();
+    const chromo_replica *R = &Rs;
     if (R->counters[CHROMO_CNT_ERROR] != 0) return;
     if (R->B < 1 || R->B + 1 > max_bp1) {
         counters_error(R, ERR_NBINDERS);

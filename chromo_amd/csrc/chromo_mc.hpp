@@ -148,6 +148,18 @@ __device__ inline Arena make_arena(char *base, const ArenaLayout &L)
 
 constexpr size_t MT_LDS_BYTES = 624 * sizeof(uint32_t);
 
+// Copies a replica record into LDS, all threads of the workgroup helping; the
+// caller synchronises.  The kernels then read the record's fields from LDS: read
+// from HBM through the table pointer they would be loaded again after every store
+// the compiler cannot tell apart from the table.
+__device__ __forceinline__ void load_record(chromo_replica *dst, const chromo_replica *src)
+{
+    static_assert(sizeof(chromo_replica) % 4 == 0, "word copy");
+    const uint32_t *s = (const uint32_t *)src;
+    uint32_t *d = (uint32_t *)dst;
+    for (int i = (int)threadIdx.x; i < (int)(sizeof(chromo_replica) / 4); i += (int)blockDim.x) d[i] = s[i];
+}
+
 // Dynamic LDS of the field-dE tape kernel: the LDS move arena and an MT block.
 inline size_t lds_bytes_for(int Bp1)
 {

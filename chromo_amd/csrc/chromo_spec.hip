@@ -24,7 +24,10 @@ k_mc_spec(const chromo_replica *__restrict__ reps, int n_reps, int64_t steps)
 {
     const int r = blockIdx.x;
     if (r >= n_reps) return;
-    const chromo_replica *R = &reps[r];
+    __shared__ chromo_replica Rs;
+    load_record(&Rs, &reps[r]);
+    __syncthreads();
+    const chromo_replica *R = &Rs;
     if (R->B != NB) return;
     if (R->counters[CHROMO_CNT_ERROR] != 0) return;
     spec_run<NB, W>(R, steps);
