@@ -229,18 +229,19 @@ struct WaveRng {
     int mode;
     uint32_t g;
     int gf, gr;
-    uint32_t *mt;       // 624 words in LDS (numpy MT19937 state)
+    uint32_t *mt;       // 624 words in LDS: numpy MT19937 state (glibc mode) or the Philox batch
     int mti;
-    // Philox: a batch of WAVE consecutive blocks, block pbase + j on lane j (words
-    // p0..p3); pidx is the next of its 4 * WAVE words.  pbase == PHILOX_NONE: no
-    // batch yet.
+    // Philox: a batch of WAVE consecutive blocks in LDS, block pbase + j at words
+    // 4j .. 4j + 3 of mt, its base counter at words PHILOX_BASE.. (shared by every
+    // WaveRng of the wavefront that uses this LDS); pidx is the next of its 4 * WAVE
+    // words.  pbase == PHILOX_NONE: no batch.
     uint32_t k0, k1;
     uint64_t pbase;
-    uint32_t p0, p1, p2, p3;
     int pidx;
 };
 
 constexpr uint64_t PHILOX_NONE = ~0ull;
+constexpr int PHILOX_BASE = 4 * 64;     // words of the batch, then its base counter
 
 __device__ __forceinline__ int32_t glibc_draw(WaveRng &R)
 {
@@ -279,18 +280,22 @@ __device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1
     }
 }
 
-// Blocks base .. base + WAVE - 1 of the stream, one per lane: the same words a
-// block-at-a-time generator produces, WAVE blocks per Philox latency.
+// Blocks base .. base + WAVE - 1 of the stream, one per lane, into the LDS batch:
+// the same words a block-at-a-time generator produces, WAVE blocks per Philox
+// latency.
 __device__ __forceinline__ void philox_fill(WaveRng &R, uint64_t base)
 {
     R.pbase = uni_u64(base);
-    const uint64_t c = R.pbase + (uint64_t)lane_id();
+    const int lane = lane_id();
+    const uint64_t c = R.pbase + (uint64_t)lane;
     uint32_t w[4] = {(uint32_t)c, (uint32_t)(c >> 32), 0u, 0u};
     philox10(w, uni_u32(R.k0), uni_u32(R.k1));
-    R.p0 = w[0];
-    R.p1 = w[1];
-    R.p2 = w[2];
-    R.p3 = w[3];
+    *(uint4 *)&R.mt[4 * lane] = make_uint4(w[0], w[1], w[2], w[3]);
+    if (lane == 0) {
+        R.mt[PHILOX_BASE] = (uint32_t)R.pbase;
+        R.mt[PHILOX_BASE + 1] = (uint32_t)(R.pbase >> 32);
+    }
+    wave_sync();
 }
 
 __device__ __forceinline__ uint32_t philox_draw(WaveRng &R)
@@ -300,11 +305,19 @@ __device__ __forceinline__ uint32_t philox_draw(WaveRng &R)
         philox_fill(R, R.pbase + WAVE);
         R.pidx = 0;
     }
-    const int w = R.pidx & 3;
-    const uint32_t x = w == 0 ? R.p0 : (w == 1 ? R.p1 : (w == 2 ? R.p2 : R.p3));
-    const uint32_t v = readlane_u32(x, R.pidx >> 2);
+    const uint32_t v = uni_u32(R.mt[R.pidx]);
     R.pidx++;
     return v;
+}
+
+// Marks the wavefront's LDS batch empty (before a WaveRng first uses it).
+__device__ __forceinline__ void philox_reset(const WaveRng &R)
+{
+    if (lane_id() == 0) {
+        R.mt[PHILOX_BASE] = (uint32_t)PHILOX_NONE;
+        R.mt[PHILOX_BASE + 1] = (uint32_t)(PHILOX_NONE >> 32);
+    }
+    wave_sync();
 }
 
 // The block-at-a-time form of the Philox position, as stored between launches and
@@ -321,26 +334,26 @@ __device__ __forceinline__ void philox_get(const WaveRng &R, uint64_t &ctr, int 
         const int l = R.pidx >> 2;
         ctr = blk + 1;
         bi = R.pidx & 3;
-        b[0] = readlane_u32(R.p0, l);
-        b[1] = readlane_u32(R.p1, l);
-        b[2] = readlane_u32(R.p2, l);
-        b[3] = readlane_u32(R.p3, l);
+        for (int j = 0; j < 4; ++j) b[j] = uni_u32(R.mt[4 * l + j]);
     }
 }
 
-// Moves to the position (ctr, bi) of philox_get; recomputes the batch only when
-// that position lies outside it.
+// Moves to the position (ctr, bi) of philox_get; recomputes the LDS batch only
+// when that position lies outside the batch it holds.
 __device__ __forceinline__ void philox_set(WaveRng &R, uint64_t ctr, int bi)
 {
     ctr = uni_u64(ctr);
     bi = uni_i32(bi);
     const uint64_t blk = bi >= 4 ? ctr : ctr - 1;
     const int w = bi >= 4 ? 0 : bi;
-    if (R.pbase == PHILOX_NONE || blk < R.pbase || blk >= R.pbase + WAVE) {
+    wave_sync();
+    const uint64_t have = uni_u64((uint64_t)R.mt[PHILOX_BASE] | ((uint64_t)R.mt[PHILOX_BASE + 1] << 32));
+    if (have == PHILOX_NONE || blk < have || blk >= have + WAVE) {
         philox_fill(R, blk);
         R.pidx = w;
     } else {
-        R.pidx = (int)(blk - R.pbase) * 4 + w;
+        R.pbase = have;
+        R.pidx = (int)(blk - have) * 4 + w;
     }
 }
 
@@ -388,14 +401,7 @@ __device__ __forceinline__ double rnd_unit(WaveRng &R) { return (double)rnd31(R)
 // draw); valid while pidx + t < 4 * WAVE.  Advances nothing.
 __device__ __forceinline__ uint32_t philox_peek(const WaveRng &R, int t)
 {
-    const int idx = R.pidx + t;
-    const int addr = (idx >> 2) << 2;   // byte address of the source lane
-    const int w = idx & 3;
-    const uint32_t a = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)R.p0);
-    const uint32_t b = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)R.p1);
-    const uint32_t c = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)R.p2);
-    const uint32_t d = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)R.p3);
-    return w == 0 ? a : (w == 1 ? b : (w == 2 ? c : d));
+    return R.mt[R.pidx + t];
 }
 
 // C's x % m for a draw x in [0, 2^31) and m >= 1, in 32-bit arithmetic.

@@ -951,10 +951,11 @@ __device__ __forceinline__ void rng_load(const chromo_replica *R, WaveRng &rng, 
     rng.k1 = uni_u32(w[CHROMO_RNG_PHILOX_KEY + 1]);
     rng.pbase = PHILOX_NONE;
     rng.pidx = 0;
-    rng.p0 = rng.p1 = rng.p2 = rng.p3 = 0;
-    if (rng.mode == CHROMO_RNG_PHILOX)
+    if (rng.mode == CHROMO_RNG_PHILOX) {
+        philox_reset(rng);
         philox_set(rng, (uint64_t)w[CHROMO_RNG_PHILOX_CTR] | ((uint64_t)w[CHROMO_RNG_PHILOX_CTR + 1] << 32),
                    (int)w[CHROMO_RNG_PHILOX_IDX]);
+    }
     wave_sync();
 }
 

@@ -300,9 +300,8 @@ __device__ __forceinline__ void seq_step(const chromo_replica *R, SeqIO *io, uin
     rng.k1 = uni_u32(R->rng[CHROMO_RNG_PHILOX_KEY + 1]);
     rng.pbase = PHILOX_NONE;
     rng.pidx = 0;
-    rng.p0 = rng.p1 = rng.p2 = rng.p3 = 0;
+    rng.mt = mt;      // the wavefront's LDS batch: rng_restore reuses it when it can
     rng_restore(rng, io->ck);
-    rng.mt = mt;
     rng.mti = uni_i32(io->mti);
     chromo_ctrl C = io->C;
     MoveOut o;
