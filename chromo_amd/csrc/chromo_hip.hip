@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(64) k_mc_run(const chromo_replica *__restrict_
     const int Bp1 = R->B + 1;
     const SegLds G = seg_lds(lds_dyn, Bp1);
     seg_clear(G);
-    uint32_t *mt_lds = (uint32_t *)(lds_dyn + seg_layout<SEG_WIDE>(Bp1).total);
+    uint32_t *mt_lds = (uint32_t *)(lds_dyn + wide_lds_bytes(Bp1));
     const bool need_mt = any_binding(R);
     WaveRng rng;
     rng_load(R, rng, mt_lds, need_mt);
@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(64) k_mc_step(const chromo_replica *__restrict
     const int Bp1 = R->B + 1;
     const SegLds G = seg_lds(lds_dyn, Bp1);
     seg_clear(G);
-    uint32_t *mt_lds = (uint32_t *)(lds_dyn + seg_layout<SEG_WIDE>(Bp1).total);
+    uint32_t *mt_lds = (uint32_t *)(lds_dyn + wide_lds_bytes(Bp1));
     chromo_ctrl C = R->ctrl[ci];
     const bool need_mt = (C.move == CHROMO_CHANGE_BINDING_STATE);
     WaveRng rng;

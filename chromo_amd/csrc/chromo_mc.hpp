@@ -36,6 +36,7 @@ struct Proposal {
 }  // namespace
 
 #include "chromo_seg.hpp"
+#include "chromo_big.hpp"
 
 namespace {
 
@@ -170,7 +171,15 @@ inline size_t lds_bytes_for(int Bp1)
 // Dynamic LDS of the one-wavefront Monte Carlo kernels: one evaluator block for
 // moves of up to SEG_WIDE beads (its start doubles as the SEG_MAX block), then the
 // MT19937 state.
-inline size_t mc_lds_bytes(int Bp1) { return seg_layout<SEG_WIDE>(Bp1).total + MT_LDS_BYTES; }
+// The wide block of the one-at-a-time path: a SEG_WIDE evaluator block or two
+// SEG_MAX ones, whichever is larger; the long-move evaluator uses all of it.
+__host__ __device__ inline size_t wide_lds_bytes(int Bp1)
+{
+    const size_t a = seg_layout<SEG_WIDE>(Bp1).total, b = 2 * seg_layout<SEG_MAX>(Bp1).total;
+    return a > b ? a : b;
+}
+
+inline size_t mc_lds_bytes(int Bp1) { return wide_lds_bytes(Bp1) + MT_LDS_BYTES; }
 
 extern __shared__ char lds_dyn[];
 
@@ -857,6 +866,17 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
             // the wide block covered the SEG_MAX blocks: their hashes must be empty again
             for (int b = 0; b < nstd; ++b)
                 seg_clear<SEG_MAX>(seg_lds<SEG_MAX>(wide + (size_t)b * seg_layout<SEG_MAX>(NB + 1).total, NB + 1));
+        } else if (wide != nullptr && big_move(P.move)) {
+            const BigLds GB = big_lds(wide, NB + 1, wide_lds_bytes(NB + 1));
+            if (GB.ucap > 0) {
+                big_eval<NB>(R, P, u, GB, sr);
+                if (!sr.overflow) {
+                    if (sr.accepted) big_commit<NB>(R, P, GB, sr.U);
+                    wave_sync_global();
+                }
+                for (int b = 0; b < nstd; ++b)
+                    seg_clear<SEG_MAX>(seg_lds<SEG_MAX>(wide + (size_t)b * seg_layout<SEG_MAX>(NB + 1).total, NB + 1));
+            }
         }
         if (!sr.overflow) {
             out.dEp = sr.dEp;

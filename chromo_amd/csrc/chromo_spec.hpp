@@ -105,7 +105,7 @@ struct SpecLds {
     uint32_t *mt;
     SegLds mine;    // this wavefront's evaluator block
     SegLds first;   // wavefront 0's (the one-at-a-time path)
-    char *wide;     // the W blocks as one SEG_WIDE block, or nullptr if they are too small
+    char *wide;     // the W blocks as one wide block (wide_lds_bytes), or nullptr if they are too small
 };
 __device__ __forceinline__ void rng_save(const WaveRng &g, uint32_t *ck)
 {
@@ -742,7 +742,7 @@ __device__ __forceinline__ void spec_run(const chromo_replica *R, int64_t steps)
     SpecLds L;
     L.mine = seg_lds(lds_dyn + SL.seg + SL.seg_bytes * wv, Bp1);
     L.first = seg_lds(lds_dyn + SL.seg, Bp1);
-    L.wide = SL.seg_bytes * W >= seg_layout<SEG_WIDE>(Bp1).total ? lds_dyn + SL.seg : nullptr;
+    L.wide = SL.seg_bytes * W >= wide_lds_bytes(Bp1) ? lds_dyn + SL.seg : nullptr;
     seg_clear(L.mine);
     L.slots[0] = (SpecSlot *)(lds_dyn + SL.slots);
     L.slots[1] = L.slots[0] + W;

@@ -191,9 +191,10 @@ def test_mc_sim_one_and_four_binders(oracle, B, waves):
 
 
 @pytest.mark.parametrize("waves", [1, 2, P4])
-def test_mc_sim_large_moves_use_global_arena(oracle, waves):
-    """Crank and slide windows up to 150 beads: moves above 32 beads run through
-    the global scratch arena (and, with several wavefronts, one at a time)."""
+def test_mc_sim_large_moves(oracle, waves):
+    """Crank, slide and end-pivot windows up to 150 beads: moves above 32 beads run
+    one at a time, up to 64 beads in the wide evaluator block and longer ones in
+    the chunked LDS evaluator."""
     say("mc_sim large moves")
     s = W.make_system(1500, 2, 16, seed=23)
     for c in s.ctrls:
@@ -202,6 +203,26 @@ def test_mc_sim_large_moves_use_global_arena(oracle, waves):
             c.bead_lo = c.bead_hi
     lo = _compare_run(oracle, s, steps=6, waves=waves)
     assert lo["n"].max() > 32
+
+
+@pytest.mark.parametrize("nvox,voxel,waves", [(1, W.VOXEL, 1), (2, W.VOXEL, 2), (40, 10.0, 1), (40, 10.0, 2),
+                                              (64, 2.0, 2)])
+def test_mc_sim_long_moves_tiny_grids_and_overflow(oracle, nvox, voxel, waves):
+    """Windows up to 150 beads on grids of one and two voxels per side (corners
+    coincide or wrap), and on 10 nm and 2 nm voxels, where long moves touch
+    hundreds to thousands of voxels: those beyond what the LDS evaluator holds
+    go through the global arena."""
+    say("mc_sim long moves", nvox, voxel)
+    s = W.make_system(1500, 2, nvox, seed=28, chi=0.1, voxel=voxel)
+    s.field.vf_limit = 1e9
+    for c in s.ctrls:
+        if c.move in (L.CRANK_SHAFT, L.SLIDE, L.END_PIVOT):
+            c.amp_bead = int(c.bead_hi)
+            c.bead_lo = c.bead_hi
+    lo = _compare_run(oracle, s, steps=4, waves=waves)
+    assert lo["n"].max() > 64
+    if voxel < W.VOXEL:
+        assert lo["n_bins"].max() > 1000
 
 
 @pytest.mark.parametrize("waves", [2, P4, pytest.param(8, marks=pytest.mark.skip(reason=W8_SKIP))])
