@@ -187,22 +187,103 @@ __device__ __forceinline__ void seg_transform(const Proposal &P, const double o[
 
 __device__ __forceinline__ int wrap1(int c, int nn) { return c + 1 == nn ? 0 : c + 1; }
 
+// What an evaluation reads of the chain, loaded in one batch before any of its
+// arithmetic: lane i's bead (window bead ind0 + i, or tangent pick i), its
+// coordinates, binder states and (binding moves) modifications, and the r, t3
+// and bond parameters of the neighbours the polymer dE needs: beads ind0 - 1 and
+// indf on lanes 0 and n - 1, beads b - 1 and b + 1 of every tangent pick.
+template <int NB>
+struct SegBeads {
+    int64_t b;
+    double o[9];
+    int8_t so[NB], mods[NB];
+    double lr[6], rr[6];   // r, t3 of beads b - 1 and b + 1
+    Bond lb, rb;           // bonds (b - 1, b) and (b, b + 1)
+    bool hasl, hasr;
+};
+
+template <int NB>
+__device__ __forceinline__ void seg_load(const chromo_replica *R, int mv, int64_t n, int64_t ind0, int64_t indf,
+                                         int64_t pick, SegBeads<NB> &B)
+{
+    const int lane = lane_id();
+    const int64_t N = R->N;
+    const GLOBAL_AS double *Pp = gmem(R->pos);
+    const bool tangent = mv == CHROMO_TANGENT_ROTATION;
+    const bool binding = mv == CHROMO_CHANGE_BINDING_STATE;
+    const bool act = lane < n;
+    int64_t b = tangent ? pick : ind0 + lane;
+    if (!act) b = tangent ? 0 : ind0;
+    B.b = b;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) B.o[c] = 0;
+#pragma unroll
+    for (int m = 0; m < NB; ++m) B.so[m] = B.mods[m] = 0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) B.lr[c] = B.rr[c] = 0;
+    B.lb = Bond{0, 0, 0, 0, 0};
+    B.rb = Bond{0, 0, 0, 0, 0};
+    if (act) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) B.o[c] = Pp[c * N + b];
+#pragma unroll
+        for (int m = 0; m < NB; ++m) B.so[m] = gmem(R->states)[(int64_t)m * N + b];
+        if (binding) {
+#pragma unroll
+            for (int m = 0; m < NB; ++m) B.mods[m] = gmem(R->marks)[(int64_t)m * N + b];
+        }
+    }
+    B.hasl = act && !binding && (tangent ? b != 0 : (lane == 0 && ind0 != 0));
+    B.hasr = act && !binding && (tangent ? b + 1 != N : (lane == n - 1 && indf != N));
+    if (B.hasl) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            B.lr[j] = Pp[j * N + b - 1];
+            B.lr[3 + j] = Pp[(3 + j) * N + b - 1];
+        }
+        B.lb = load_bond(R->bonds, N, b - 1);
+    }
+    if (B.hasr) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            B.rr[j] = Pp[j * N + b + 1];
+            B.rr[3 + j] = Pp[(3 + j) * N + b + 1];
+        }
+        B.rb = load_bond(R->bonds, N, b);
+    }
+}
+
+// r of bead x (wave-uniform, ind0 - 1 <= x <= indf, a bead seg_load fetched) of a
+// non-tangent move of n beads.
+template <int NB>
+__device__ __forceinline__ void seg_r3(const SegBeads<NB> &B, int64_t x, int64_t ind0, int64_t indf, int64_t n,
+                                       double v[3])
+{
+    if (x == ind0 - 1) {
+        for (int j = 0; j < 3; ++j) v[j] = readlane_f64(B.lr[j], 0);
+    } else if (x == indf) {
+        for (int j = 0; j < 3; ++j) v[j] = readlane_f64(B.rr[j], (int)(n - 1));
+    } else {
+        for (int j = 0; j < 3; ++j) v[j] = readlane_f64(B.o[j], (int)(x - ind0));
+    }
+}
+
 // Evaluates proposal P (1 <= n <= SMAX) against the committed state: polymer
 // dE (SSWLC.compute_dE, polymers.pyx:1024-1082), field dE
 // (UniformDensityField.compute_dE, fields.pyx:1149-1233) and the Metropolis test
 // with the uniform draw u (mc_sim.pyx:152-182).  Writes nothing outside S and K.
-// Called by one whole wavefront; S's hash is empty on entry and on return.
+// Called by one whole wavefront; S's hash is empty on entry and on return.  B
+// holds what seg_load fetched for this move.
 template <int NB, int SMAX = SEG_MAX>
-__device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal &P, double u, const SegLds &S,
-                                         SegRes &res, SegKeep<NB> &K)
+__device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const Proposal &P, double u,
+                                                const SegLds &S, SegRes &res, SegKeep<NB> &K,
+                                                const SegBeads<NB> &B)
 {
     using C = SegCap<SMAX>;
     constexpr int MW = C::MW;
     constexpr int Bp1 = NB + 1;
     const int lane = lane_id();
     const int64_t N = R->N;
-    const GLOBAL_AS double *Pp = gmem(R->pos);
-    const GLOBAL_AS int8_t *states = gmem(R->states);
     const int mv = P.move;
     const int64_t n = P.n, ind0 = P.ind0, indf = P.indf;
     const bool tangent = mv == CHROMO_TANGENT_ROTATION;
@@ -212,28 +293,17 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
     res.overflow = 0;
 
     // ---- bead i == lane: coordinates, states, trial
-    int64_t b = tangent ? P.mypick : ind0 + lane;
-    if (!act) b = tangent ? 0 : ind0;
-    K.b = b;
+    K.b = B.b;
     double o[9];
     int8_t so[NB], sn[NB];
 #pragma unroll
-    for (int c = 0; c < 9; ++c) o[c] = 0;
+    for (int c = 0; c < 9; ++c) o[c] = B.o[c];
 #pragma unroll
-    for (int m = 0; m < NB; ++m) so[m] = sn[m] = 0;
-    if (act) {
+    for (int m = 0; m < NB; ++m) so[m] = sn[m] = B.so[m];
+    if (act && state_change) {
 #pragma unroll
-        for (int c = 0; c < 9; ++c) o[c] = Pp[c * N + b];
-#pragma unroll
-        for (int m = 0; m < NB; ++m) {
-            so[m] = states[(int64_t)m * N + b];
-            sn[m] = so[m];
-        }
-        if (state_change) {
-#pragma unroll
-            for (int m = 0; m < NB; ++m)
-                if (m == P.binder) sn[m] = (int8_t)P.mynew;
-        }
+        for (int m = 0; m < NB; ++m)
+            if (m == P.binder) sn[m] = (int8_t)P.mynew;
     }
     double w[9];
     seg_transform(P, o, w);
@@ -245,51 +315,16 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
     if (tangent || state_change) {
         double e = 0;
         if (act && tangent) {
-            if (b != 0) {
-                double r0[3], t30[3];
-                for (int j = 0; j < 3; ++j) {
-                    r0[j] = Pp[j * N + b - 1];
-                    t30[j] = Pp[(3 + j) * N + b - 1];
-                }
-                Bond bd = load_bond(R->bonds, N, b - 1);
-                e += pair_dE_forward(bd, r0, &o[0], &w[0], t30, &o[3], &w[3]);
-            }
-            if (b + 1 != N) {
-                double r1[3], t31[3];
-                for (int j = 0; j < 3; ++j) {
-                    r1[j] = Pp[j * N + b + 1];
-                    t31[j] = Pp[(3 + j) * N + b + 1];
-                }
-                Bond bd = load_bond(R->bonds, N, b);
-                e += pair_dE_reverse(bd, &o[0], &w[0], r1, &o[3], &w[3], t31);
-            }
+            if (B.hasl) e += pair_dE_forward(B.lb, &B.lr[0], &o[0], &w[0], &B.lr[3], &o[3], &w[3]);
+            if (B.hasr) e += pair_dE_reverse(B.rb, &o[0], &w[0], &B.rr[0], &o[3], &w[3], &B.rr[3]);
         } else if (act) {
-            int8_t mods[NB];
-#pragma unroll
-            for (int m = 0; m < NB; ++m) mods[m] = gmem(R->marks)[(int64_t)m * N + b];
-            e = bead_binding_dE<NB>(R, mods, so, sn);
+            e = bead_binding_dE<NB>(R, B.mods, so, sn);
         }
         for (int i = 0; i < n; ++i) dEp += readlane_f64(e, i);
     } else {
         double ef = 0, er = 0;
-        if (lane == 0 && ind0 != 0) {
-            double r0[3], t30[3];
-            for (int j = 0; j < 3; ++j) {
-                r0[j] = Pp[j * N + ind0 - 1];
-                t30[j] = Pp[(3 + j) * N + ind0 - 1];
-            }
-            Bond bd = load_bond(R->bonds, N, ind0 - 1);
-            ef = pair_dE_forward(bd, r0, &o[0], &w[0], t30, &o[3], &w[3]);
-        }
-        if (lane == n - 1 && indf != N) {
-            double r1[3], t31[3];
-            for (int j = 0; j < 3; ++j) {
-                r1[j] = Pp[j * N + indf];
-                t31[j] = Pp[(3 + j) * N + indf];
-            }
-            Bond bd = load_bond(R->bonds, N, indf - 1);
-            er = pair_dE_reverse(bd, &o[0], &w[0], r1, &o[3], &w[3], t31);
-        }
+        if (lane == 0 && ind0 != 0) ef = pair_dE_forward(B.lb, &B.lr[0], &o[0], &w[0], &B.lr[3], &o[3], &w[3]);
+        if (lane == n - 1 && indf != N) er = pair_dE_reverse(B.rb, &o[0], &w[0], &B.rr[0], &o[3], &w[3], &B.rr[3]);
         if (ind0 != 0) dEp += readlane_f64(ef, 0);
         if (indf != N) dEp += readlane_f64(er, (int)(n - 1));
     }
@@ -515,6 +550,16 @@ __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal
     res.expdE = exp(-dE);
     res.accepted = u < res.expdE;
     res.U = U;
+}
+
+// seg_eval_loaded with its own loads (the one-at-a-time path).
+template <int NB, int SMAX = SEG_MAX>
+__device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal &P, double u, const SegLds &S,
+                                         SegRes &res, SegKeep<NB> &K)
+{
+    SegBeads<NB> B;
+    seg_load<NB>(R, P.move, P.n, P.ind0, P.indf, P.mypick, B);
+    seg_eval_loaded<NB, SMAX>(R, P, u, S, res, K, B);
 }
 
 // Writes an accepted proposal: bead coordinates or states (MCAdapter.accept,

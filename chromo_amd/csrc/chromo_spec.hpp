@@ -316,15 +316,16 @@ __device__ __forceinline__ void seq_step(const chromo_replica *R, SeqIO *io, uin
 }
 
 // The proposal of slot S from its stored draws: the arithmetic of mc_step_dev's
-// proposal part (move_funcs.pyx).  False when a crank-shaft's axis beads
-// coincide: the reference then draws a random axis, which only the one-at-a-time
-// path can do in stream order.
+// proposal part (move_funcs.pyx).  The axis and fulcrum beads lie in the window or
+// next to it: their coordinates come from B.  False when a crank-shaft's axis
+// beads coincide: the reference then draws a random axis, which only the
+// one-at-a-time path can do in stream order.
 template <int NB>
-__device__ __forceinline__ bool slot_proposal(const chromo_replica *R, const SpecSlot &S, Proposal &P)
+__device__ __forceinline__ bool slot_proposal(const chromo_replica *R, const SpecSlot &S, Proposal &P,
+                                              const SegBeads<NB> &B)
 {
     const int lane = lane_id();
     const int64_t N = R->N;
-    const double *Pp = R->pos;
     const int mv = S.move;
     const int64_t n = S.n, ind0 = S.ind0, indf = S.indf;
     P.move = mv;
@@ -354,9 +355,9 @@ __device__ __forceinline__ bool slot_proposal(const chromo_replica *R, const Spe
         else if (ind0 == 0 && indf == N) ifc = ind0;
         else ifc = ind0_m1;
         double ra[3], rb[3], pt[3], dir[3];
-        load_pos3(Pp, N, ia, ra);
-        load_pos3(Pp, N, ib, rb);
-        load_pos3(Pp, N, ifc, pt);
+        seg_r3(B, ia, ind0, indf, n, ra);
+        seg_r3(B, ib, ind0, indf, n, rb);
+        seg_r3(B, ifc, ind0, indf, n, pt);
         for (int i = 0; i < 3; ++i) dir[i] = ra[i] - rb[i];
         double mag = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
         if (mag < 1E-5) return false;
@@ -371,7 +372,7 @@ __device__ __forceinline__ bool slot_proposal(const chromo_replica *R, const Spe
         else if (ind0 == 0 && indf == N && S.lhs == 1) ifc = indf - 1;
         else ifc = ind0;
         double pt[3], axis[3];
-        load_pos3(Pp, N, ifc, pt);
+        seg_r3(B, ifc, ind0, indf, n, pt);
         sphere_from(S.s1, S.s2, axis);
         axis_rotation(axis, pt, S.ang, P.T);
     } else if (mv == CHROMO_SLIDE) {
@@ -417,15 +418,19 @@ __device__ __forceinline__ bool eval_slot(const chromo_replica *R, const SpecSlo
         res.wlo = ind0;
         res.whi = indf;
     }
+    // every load of the evaluation but the densities', issued before its arithmetic
+    SegBeads<NB> B;
+    const int64_t pick = (S.move == CHROMO_TANGENT_ROTATION && lane < S.n) ? S.picks[lane] : 0;
+    seg_load<NB>(R, S.move, S.n, ind0, indf, pick, B);
     Proposal P;
-    if (!slot_proposal<NB>(R, S, P)) {
+    if (!slot_proposal<NB>(R, S, P, B)) {
         if (lane == 0) res.resample = 1;
         wave_sync();
         return false;
     }
     SegRes sr;
     const double u = (double)S.ud / RMAX;
-    seg_eval<NB>(R, P, u, G, sr, K);
+    seg_eval_loaded<NB>(R, P, u, G, sr, K, B);
     if (sr.overflow) {
         // more voxels than seg_eval holds: the one-at-a-time path evaluates it
         if (lane == 0) res.overflow = 1;
