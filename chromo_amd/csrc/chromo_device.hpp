@@ -459,27 +459,63 @@ __device__ __forceinline__ int64_t rnd_randint(WaveRng &R, int64_t high)
 // ---------------------------------------------------------------------------
 
 // uniform_sample_unit_sphere, linalg.pyx:23-59, from its two rand() draws
-__device__ __forceinline__ void sphere_from(int32_t d1, int32_t d2, double v[3])
+// sin and cos of up to three wave-uniform angles, one lane each: one sin and one
+// cos latency for all three.  Called by the whole wavefront.
+__device__ __forceinline__ void sincos_uni3(double a0, double a1, double a2, double s[3], double c[3])
 {
-    double phi = (double)d1 / RMAX * (2.0 * M_PI);
-    double theta = acos(((double)d2 / RMAX) * 2 - 1);
-    v[0] = cos(phi) * sin(theta);
-    v[1] = sin(phi) * sin(theta);
-    v[2] = cos(theta);
+    const int lane = lane_id();
+    const double x = lane == 1 ? a1 : (lane == 2 ? a2 : a0);
+    const double sv = sin(x), cv = cos(x);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        s[j] = readlane_f64(sv, j);
+        c[j] = readlane_f64(cv, j);
+    }
 }
 
+__device__ __forceinline__ double sphere_phi(int32_t d1) { return (double)d1 / RMAX * (2.0 * M_PI); }
+__device__ __forceinline__ double sphere_theta(int32_t d2) { return acos(((double)d2 / RMAX) * 2 - 1); }
+
+__device__ __forceinline__ void sphere_dir(double sp, double cp, double st, double ct, double v[3])
+{
+    v[0] = cp * st;
+    v[1] = sp * st;
+    v[2] = ct;
+}
+
+// Per lane: the unit vector of the draws d1, d2.
+__device__ __forceinline__ void sphere_from(int32_t d1, int32_t d2, double v[3])
+{
+    const double phi = sphere_phi(d1), theta = sphere_theta(d2);
+    sphere_dir(sin(phi), cos(phi), sin(theta), cos(theta), v);
+}
+
+// Wave-uniform draws and angle: the unit vector and the sine and cosine of ang,
+// at once.  Called by the whole wavefront.
+__device__ __forceinline__ void sphere_from_uni(int32_t d1, int32_t d2, double ang, double v[3], double &sa,
+                                                double &ca)
+{
+    double s[3], c[3];
+    sincos_uni3(sphere_phi(d1), sphere_theta(d2), ang, s, c);
+    sphere_dir(s[0], c[0], s[1], c[1], v);
+    sa = s[2];
+    ca = c[2];
+}
+
+// Called by the whole wavefront.
 __device__ __forceinline__ void sample_unit_sphere(WaveRng &R, double v[3])
 {
     int32_t d1 = rnd31(R);
     int32_t d2 = rnd31(R);
-    sphere_from(d1, d2, v);
+    double sa, ca;
+    sphere_from_uni(d1, d2, 0.0, v, sa, ca);
 }
 
-// arbitrary_axis_rotation, linalg.pyx:62-139 (rows 0..2 of the 4x4 matrix)
-__device__ __forceinline__ void axis_rotation(const double a[3], const double p[3], double ang,
-                                              double T[12])
+// arbitrary_axis_rotation, linalg.pyx:62-139 (rows 0..2 of the 4x4 matrix), with
+// c = cos(ang) and s = sin(ang) given
+__device__ __forceinline__ void axis_rotation_cs(const double a[3], const double p[3], double c, double s,
+                                                 double T[12])
 {
-    double c = cos(ang), s = sin(ang);
     T[0] = a[0] * a[0] + (a[1] * a[1] + a[2] * a[2]) * c;
     T[1] = a[0] * a[1] * (1 - c) - a[2] * s;
     T[2] = a[0] * a[2] * (1 - c) + a[1] * s;
@@ -498,6 +534,11 @@ __device__ __forceinline__ void axis_rotation(const double a[3], const double p[
     T[3] = rv0;
     T[7] = rv1;
     T[11] = rv2;
+}
+
+__device__ __forceinline__ void axis_rotation(const double a[3], const double p[3], double ang, double T[12])
+{
+    axis_rotation_cs(a, p, cos(ang), sin(ang), T);
 }
 
 // One row of transform_r_t3_t2 (move_funcs.pyx:139-154): left fold from zero.

@@ -696,8 +696,12 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
         if (!err && indf > N) err = ERR_WINDOW;
         if (err) goto fail;
         n = indf - ind0;
-        double axis[3];
-        sample_unit_sphere(rng, axis);
+        double axis[3], sa, ca;
+        {
+            const int32_t d1 = rnd31(rng);
+            const int32_t d2 = rnd31(rng);
+            sphere_from_uni(d1, d2, ang, axis, sa, ca);
+        }
         // get_end_pivot_fulcrum, move_funcs.pyx:349-398
         int64_t ifc;
         if (ind0 == 0 && indf != N) ifc = indf;
@@ -706,7 +710,7 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
         else ifc = ind0;
         double pt[3];
         load_pos3(Pp, N, ifc, pt);
-        axis_rotation(axis, pt, ang, P.T);
+        axis_rotation_cs(axis, pt, ca, sa, P.T);
     } else if (P.move == CHROMO_SLIDE) {
         // slide, move_funcs.pyx:403-465
         double amp = amp_move * ((double)rnd31(rng) / RMAX);
@@ -776,9 +780,11 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
         }
         if (err) goto fail;
         for (int64_t i = 0; i < k; ++i) {
-            double dir[3], T[12];
-            sample_unit_sphere(rng, dir);
-            axis_rotation(dir, origin, ang, T);
+            double dir[3], T[12], sa, ca;
+            const int32_t d1 = rnd31(rng);
+            const int32_t d2 = rnd31(rng);
+            sphere_from_uni(d1, d2, ang, dir, sa, ca);
+            axis_rotation_cs(dir, origin, ca, sa, T);
             if (i < WAVE) {
                 if (lane == i) {
                     for (int j = 0; j < 3; ++j)

@@ -354,6 +354,7 @@ __device__ __forceinline__ bool slot_proposal(const chromo_replica *R, const Spe
         else if (ind0 != 0 && indf == N) ifc = ind0_m1;
         else if (ind0 == 0 && indf == N) ifc = ind0;
         else ifc = ind0_m1;
+        const double sa = sin(S.ang), ca = cos(S.ang);
         double ra[3], rb[3], pt[3], dir[3];
         seg_r3(B, ia, ind0, indf, n, ra);
         seg_r3(B, ib, ind0, indf, n, rb);
@@ -363,7 +364,7 @@ __device__ __forceinline__ bool slot_proposal(const chromo_replica *R, const Spe
         if (mag < 1E-5) return false;
         double sc = 1.0 / mag;
         for (int i = 0; i < 3; ++i) dir[i] = dir[i] * sc;
-        axis_rotation(dir, pt, S.ang, P.T);
+        axis_rotation_cs(dir, pt, ca, sa, P.T);
     } else if (mv == CHROMO_END_PIVOT) {
         // get_end_pivot_fulcrum, move_funcs.pyx:349-398
         int64_t ifc;
@@ -371,24 +372,25 @@ __device__ __forceinline__ bool slot_proposal(const chromo_replica *R, const Spe
         else if (ind0 != 0 && indf == N) ifc = ind0 - 1;
         else if (ind0 == 0 && indf == N && S.lhs == 1) ifc = indf - 1;
         else ifc = ind0;
-        double pt[3], axis[3];
+        double pt[3], axis[3], sa, ca;
+        sphere_from_uni(S.s1, S.s2, S.ang, axis, sa, ca);
         seg_r3(B, ifc, ind0, indf, n, pt);
-        sphere_from(S.s1, S.s2, axis);
-        axis_rotation(axis, pt, S.ang, P.T);
+        axis_rotation_cs(axis, pt, ca, sa, P.T);
     } else if (mv == CHROMO_SLIDE) {
-        double dir[3];
-        sphere_from(S.s1, S.s2, dir);
+        double dir[3], sa, ca;
+        sphere_from_uni(S.s1, S.s2, 0.0, dir, sa, ca);
         for (int i = 0; i < 3; ++i) dir[i] *= S.ang;
         P.T[3] = dir[0];
         P.T[7] = dir[1];
         P.T[11] = dir[2];
     } else if (mv == CHROMO_TANGENT_ROTATION) {
+        const double sa = sin(S.ang), ca = cos(S.ang);
         if (lane < n) {
             double dir[3], T[12];
             const double origin[3] = {0., 0., 0.};
             P.mypick = S.picks[lane];
             sphere_from(S.ps1[lane], S.ps2[lane], dir);
-            axis_rotation(dir, origin, S.ang, T);
+            axis_rotation_cs(dir, origin, ca, sa, T);
             for (int j = 0; j < 3; ++j)
                 for (int q = 0; q < 3; ++q) P.myT[3 * j + q] = T[4 * j + q];
         }
