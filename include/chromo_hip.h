@@ -92,6 +92,18 @@ enum chromo_counter {
     CHROMO_CNT_WORDS = 8
 };
 
+/* Values of counters[CHROMO_CNT_ERROR].  A replica whose counter is non-zero is
+ * left unchanged by later launches. */
+enum chromo_replica_error {
+    CHROMO_ERR_WINDOW = 1,      /* bead window larger than the chain (the reference's ValueError) */
+    CHROMO_ERR_SEGCAP = 2,      /* move longer than the replica's scratch arena (bead_cap / seg_cap) */
+    CHROMO_ERR_TANGENT_K = 3,   /* more distinct tangent beads than the chain has */
+    CHROMO_ERR_NBINDERS = 4,    /* binder count outside 1..4 */
+    CHROMO_ERR_SPIN = 5,        /* a rejection or probe loop passed its bound (amplitude below one bead:
+                                   the reference loops for ever) */
+    CHROMO_ERR_ARENA = 6        /* a scratch-arena index check failed */
+};
+
 /* Everything the kernels need about one replica (one chain in one field).  The
  * host fills an array of these and copies it to the device; the kernels read it
  * there.  Layout: chain arrays are structure-of-arrays for coalesced access. */

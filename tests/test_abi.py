@@ -51,3 +51,17 @@ def test_scratch_sizes_and_errors():
     buf = ctypes.create_string_buffer(256)
     h.chromo_last_error(buf, 256)
     assert b"NULL" in buf.value
+
+
+def test_replica_error_codes_agree():
+    """The header's replica error codes, the device constants and the host's
+    messages name the same codes."""
+    from chromo_amd import _lib
+    hdr = (ROOT / "include" / "chromo_hip.h").read_text()
+    block = hdr[hdr.index("enum chromo_replica_error"):]
+    block = block[:block.index("};")]
+    header = {int(v): n for n, v in re.findall(r"CHROMO_ERR_(\w+)\s*=\s*(\d+)", block)}
+    dev = (ROOT / "chromo_amd" / "csrc" / "chromo_device.hpp").read_text()
+    device = {int(v): n for n, v in re.findall(r"\bERR_(\w+)\s*=\s*(\d+)", dev)}
+    assert header == device
+    assert set(header) == set(_lib.ERROR_TEXT) == set(range(1, 7))
