@@ -141,6 +141,118 @@ __device__ __forceinline__ void rng_restore(WaveRng &g, const uint32_t *ck)
     if (g.mode == CHROMO_RNG_PHILOX) philox_set(g, (uint64_t)ck[33] | ((uint64_t)ck[34] << 32), (int)ck[39]);
 }
 
+// capped_exponential (bead_selection.pyx:19-66) with the one draw x: ok is
+// cleared when the reference would draw again (r > cap) or loop for ever.
+__device__ __forceinline__ int64_t cexp_once(int32_t x, int64_t window, int64_t cap, bool &ok)
+{
+    if (cap < 1) {
+        ok = false;
+        return 0;
+    }
+    const int64_t r = (int64_t)(-log10(((double)x / RMAX) + 0.00001) * (double)window * 0.45 + 1.0001);
+    if (r > cap) ok = false;
+    return r;
+}
+
+// from_point (bead_selection.pyx) with the side draw x_side and one exponential
+// draw x_exp.
+__device__ __forceinline__ int64_t from_point_once(int32_t x_side, int32_t x_exp, int64_t w, int64_t nb,
+                                                   int64_t ind0, bool &ok)
+{
+    if (w > nb || w < 1) {
+        ok = false;
+        return 0;
+    }
+    if ((x_side & 1) == 0) {
+        const int64_t ws = imax64(imin64(w, ind0), 1), ub = imax64(ind0, 1);
+        if (ws > ub) ok = false;
+        return ub - cexp_once(x_exp, ws, ws, ok);
+    }
+    const int64_t ws = imax64(imin64(w, nb - ind0), 1), ub = imax64(nb - ind0, 1);
+    if (ws > ub) ok = false;
+    return cexp_once(x_exp, ws, ws, ok) + ind0;
+}
+
+// Philox, crank-shaft / end-pivot / slide: the usual proposal -- every capped
+// exponential accepted at its first draw, a window of 1..SPEC_NMAX beads, no extra
+// axis draws -- from the next eight words of the batch, read in one LDS round
+// trip instead of one per draw.  The same words, in the same order and with the
+// same arithmetic, as gen_slot's draw-at-a-time code.  Consumes nothing and returns
+// false for any other case, which gen_slot then draws one word at a time.
+template <int NB>
+__device__ __forceinline__ bool gen_fast(const chromo_replica *R, WaveRng &rng, const chromo_ctrl &C, SpecSlot &S)
+{
+    const int mv = C.move;
+    if (rng.mode != CHROMO_RNG_PHILOX ||
+        (mv != CHROMO_CRANK_SHAFT && mv != CHROMO_END_PIVOT && mv != CHROMO_SLIDE))
+        return false;
+    const int p = uni_i32(rng.pidx);
+    if (p + 8 > 4 * WAVE) return false;
+    int32_t d[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) d[t] = (int32_t)(uni_u32(rng.mt[p + t]) >> 1);
+    const int64_t N = R->N;
+    const double amp_move = C.amp_move;
+    const int64_t amp_bead = C.amp_bead;
+    bool ok = true;
+    int used = 0;
+    int64_t ind0 = 0, indf = 0;
+    double ang = 0;
+    int32_t s1 = 0, s2 = 0, lhs = 0, ud = 0;
+    if (mv == CHROMO_CRANK_SHAFT) {
+        ang = amp_move * (((double)d[0] / RMAX) - 0.5);
+        const int64_t b0 = (int64_t)(int32_t)(((double)d[1] / RMAX) * (double)N);
+        const int64_t b1 = imax64(from_point_once(d[2], d[3], amp_bead, N, b0, ok), 1);
+        check_bead_bounds(b0, b1, N, ind0, indf);
+        ud = d[4];
+        used = 5;
+    } else if (mv == CHROMO_END_PIVOT) {
+        ang = amp_move * (((double)d[0] / RMAX) - 0.5);
+        lhs = (int32_t)(d[1] & 1);
+        if (amp_bead > N) ok = false;
+        const int64_t r = cexp_once(d[2], amp_bead, amp_bead, ok);
+        if (lhs == 1) {
+            ind0 = 0;
+            indf = r + 1;
+        } else {
+            ind0 = N - r;
+            indf = N;
+        }
+        if (indf > N) ok = false;
+        s1 = d[3];
+        s2 = d[4];
+        ud = d[5];
+        used = 6;
+    } else {
+        ang = amp_move * ((double)d[0] / RMAX);
+        s1 = d[1];
+        s2 = d[2];
+        const int64_t b0 = draw_mod(d[3], N);
+        const int64_t b1 = from_point_once(d[4], d[5], amp_bead, N, b0, ok);
+        check_bead_bounds(b0, b1, N, ind0, indf);
+        ud = d[6];
+        used = 7;
+    }
+    const int64_t n = indf - ind0;
+    if (!ok || n < 1 || n > SPEC_NMAX) return false;
+    rng.pidx = p + used;
+    if (lane_id() == 0) {
+        S.kind = SLOT_SMALL;
+        S.move = mv;
+        S.binder = 0;
+        S.lhs = lhs;
+        S.ud = ud;
+        S.s1 = s1;
+        S.s2 = s2;
+        S.n = n;
+        S.ind0 = ind0;
+        S.indf = indf;
+        S.ang = ang;
+    }
+    wave_sync();
+    return true;
+}
+
 // Draws one proposal into slot S (wavefront 0), in the draw order of
 // mc_step_dev.  A proposal that is too large for one wavefront, or that fails,
 // is marked SLOT_SEQ with its draws given back: mc_step_dev redoes it.
@@ -150,6 +262,7 @@ __device__ __forceinline__ int gen_slot(const chromo_replica *R, WaveRng &rng, c
     const int lane = lane_id();
     const int64_t N = R->N;
     rng_save(rng, S.ck);
+    if (gen_fast<NB>(R, rng, C, S)) return SLOT_SMALL;
     const double amp_move = C.amp_move;
     const int64_t amp_bead = C.amp_bead;
     const int mv = C.move;
