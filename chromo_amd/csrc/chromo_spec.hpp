@@ -693,7 +693,7 @@ __device__ __forceinline__ void spec_fill(const chromo_replica *R, WaveRng &rng,
     }
 }
 
-// Wavefront 0: AcceptanceTracker updates (mc_stat.py:190-208), controller counts,
+// One wavefront: AcceptanceTracker updates (mc_stat.py:190-208), controller counts,
 // replica counters and log records of the resolved slots 0..F-1, one slot per
 // lane; the same totals as record() called once per slot in order.
 __device__ __forceinline__ void spec_book(const chromo_replica *R, chromo_ctrl &C, Counters &K,
@@ -739,18 +739,24 @@ __device__ __forceinline__ void spec_book(const chromo_replica *R, chromo_ctrl &
     }
     wave_sync();
     if (lane == 0) {
+        // every field is read before any is written: the compiler cannot tell C from
+        // K, and interleaved reads and writes would wait on LDS one at a time
+        const double alpha = C.alpha;
         double rate = C.rate;
+        const auto na = C.num_attempt, nsu = C.num_success;
+        const int64_t kp = K.prop, ka = K.acc, kn = K.sumn, kb = K.sumb, kt = K.ties, ky = K.bytes;
         for (int s = 0; s < F; ++s)
-            if ((live_m >> s) & 1) rate = (C.alpha * (double)((acc_m >> s) & 1)) + (1 - C.alpha) * rate;
+            if ((live_m >> s) & 1) rate = (alpha * (double)((acc_m >> s) & 1)) + (1 - alpha) * rate;
+        const int nacc = __popcll(acc_m);
         C.rate = rate;
-        C.num_attempt += F;
-        C.num_success += __popcll(acc_m);
-        K.prop += nlive;
-        K.acc += __popcll(acc_m);
-        K.sumn += sn;
-        K.sumb += sb;
-        K.ties += __popcll(tie_m);
-        K.bytes += sy;
+        C.num_attempt = na + F;
+        C.num_success = nsu + nacc;
+        K.prop = kp + nlive;
+        K.acc = ka + nacc;
+        K.sumn = kn + sn;
+        K.sumb = kb + sb;
+        K.ties = kt + __popcll(tie_m);
+        K.bytes = ky + sy;
         if (R->log != nullptr && logged < R->log_cap)
             K.logged = logged + nlive < R->log_cap ? logged + nlive : R->log_cap;
     }
@@ -817,24 +823,34 @@ __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng
             __syncthreads();
             if (mine && !L.res[wv].resample && !L.res[wv].overflow) spec_conflicts(cs, L.res, L, L.mine, wv);
             __syncthreads();
-            // resolution, identical in every wavefront
+            // resolution, identical in every wavefront; lane s reads slot s's result,
+            // all slots in one LDS round trip
+            uint32_t kd = SLOT_EMPTY, cmk = 0, ov = 0, rs = 0, ac = 0;
+            if (lane < cnt) {
+                const SpecRes &q = L.res[lane];
+                kd = (uint32_t)cs[lane].kind;
+                cmk = q.cmask;
+                ov = (uint32_t)q.overflow;
+                rs = (uint32_t)q.resample;
+                ac = (uint32_t)q.accepted;
+            }
             int F = cnt, rew = 0;
             uint32_t accm = 0;
             for (int s = 0; s < cnt; ++s) {
-                if (cs[s].kind == SLOT_EMPTY) continue;
-                const SpecRes &q = L.res[s];
-                if ((q.cmask & accm) || (q.overflow && s > 0)) {
+                if ((int)readlane_u32(kd, s) == SLOT_EMPTY) continue;
+                const uint32_t q_cmask = readlane_u32(cmk, s), q_overflow = readlane_u32(ov, s);
+                if ((q_cmask & accm) || (q_overflow && s > 0)) {
                     F = s;
                     break;
                 }
                 // the first slot always fits the conflict table; if it did not, the
                 // sequential path redoes it
-                if (q.resample || q.overflow) {
+                if (readlane_u32(rs, s) || q_overflow) {
                     F = s;
                     rew = 1;
                     break;
                 }
-                if (q.accepted) accm |= 1u << s;
+                if (readlane_u32(ac, s)) accm |= 1u << s;
             }
             if (mine && wv < F && L.res[wv].accepted) {
                 const SpecSlot &S = cs[wv];
@@ -842,8 +858,11 @@ __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng
                 seg_commit<NB>(R, S.move, S.n, S.binder, mynew, keep, L.mine, L.res[wv].nbins);
             }
             if (inserted) spec_release(L, L.mine, L.res[wv].nbins);
+            // the last wavefront books the round while wavefront 0 carries the
+            // unresolved slots; only the barrier below orders C and K for the
+            // next reader
+            if (wv == W - 1) spec_book(R, C, K, cs, L.res, F);
             if (wv == 0) {
-                spec_book(R, C, K, cs, L.res, F);
                 nq = spec_carry(cs, ns, F, nqa, rew);
                 remaining -= F;
             }
