@@ -100,6 +100,26 @@ def test_energy_and_gradients_match_torch():
     assert np.allclose(gt.cpu().numpy(), tht.grad.numpy(), rtol=1e-10, atol=1e-9)
 
 
+def test_energy_and_gradients_match_torch_full_size():
+    """The adjoint configuration at full size: 20,000 beads, B = 3, 48^3 voxels."""
+    say("adjoint vs torch, 20k beads")
+    from chromo_amd.energy import EnergyModel, energy
+    s = W.config_adjoint()
+    s.chain.states = np.random.RandomState(4).randint(0, 3, size=(s.chain.N, s.chain.B))
+    model = EnergyModel(s.chain, s.field, s.binders)
+    r = torch.tensor(s.chain.r, dtype=torch.float64)
+    th = torch.tensor(model.theta0, dtype=torch.float64)
+    E, gr, gt = energy(model, r.cuda(), th.cuda())
+    rt = r.clone().requires_grad_(True)
+    tht = th.clone().requires_grad_(True)
+    Et, el, fi, bi = torch_energy(s, rt, tht)
+    Et.backward()
+    E = E.cpu().numpy()
+    assert np.allclose(E, [Et.item(), el.item(), fi.item(), bi.item()], rtol=1e-11, atol=1e-8)
+    assert np.allclose(gr.cpu().numpy(), rt.grad.numpy(), rtol=1e-9, atol=1e-9)
+    assert np.allclose(gt.cpu().numpy(), tht.grad.numpy(), rtol=1e-10, atol=1e-8)
+
+
 def test_gradients_central_differences():
     say("adjoint vs central FD")
     from chromo_amd.energy import EnergyModel, energy

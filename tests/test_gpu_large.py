@@ -1,6 +1,7 @@
-"""The largest configuration at full size: 16 replicas of 10^6 beads, B = 4,
-128^3 voxels, two wavefronts per replica.  The oracle cannot run chains of this
-size in seconds, so the checks are properties that hold at any size:
+"""The large configurations at full size: 16 replicas of 10^6 beads, B = 4,
+128^3 voxels, and the 512-replica batched workload (N ~ U{2000..50000}, 32^3-64^3
+voxels).  The oracle cannot run these in seconds, so the checks are properties
+that hold at any size:
 
 * the logged dE of the accepted moves adds up to the change of the
   dE-consistent total energy (elastic + field + binding), for every move type
@@ -90,8 +91,8 @@ def test_large_config_energy_and_density(large):
         dE = float(np.sum(lg["dE_poly"][acc]) + np.sum(lg["dE_field"][acc]))
         E1 = _total_energy(_final_chain(dr, i, large[i].chain), large[i])
         assert abs(E0[i]) < 1e90 and abs(E1) < 1e90     # no volume-fraction penalty in play
-        say(f"  replica {i}: E0 {E0[i]:.6f} E1 {E1:.6f} sum dE {dE:.9f}")
-        assert abs((E1 - E0[i]) - dE) < 1e-9 * (abs(E0[i]) + abs(E1)) + 1e-6
+        say(f"  replica {i}: E0 {E0[i]:.6f} E1 {E1:.6f} E1 - E0 - sum dE = {(E1 - E0[i]) - dE:.3e}")
+        assert abs((E1 - E0[i]) - dE) < 1e-12 * (abs(E0[i]) + abs(E1)) + 1e-7
     inc = {i: _check_state(dr, i, large[i]) for i in CHECKED}
     dr.density_init()
     for i in CHECKED:
@@ -121,3 +122,42 @@ def test_large_config_all_moves(large):
             assert np.max(np.abs(inc[i] - dr.density_of(i))) < 1e-9 * one_bead
         del dr
         torch.cuda.empty_cache()
+
+
+def test_batched_512_energy_and_density():
+    """The 512-replica workload of the benchmark, 5 sweeps at two wavefronts."""
+    from chromo_amd.engine import DeviceReplicas
+    say("building the 512-replica workload")
+    systems = W.config_batched(512, seed=3)
+    sweeps = 5
+    ctrls = []
+    for s in systems:
+        cs = copy.deepcopy(s.ctrls)
+        for c in cs:
+            if c.move == L.TANGENT_ROTATION:
+                c.num_per_cycle = 0
+        ctrls.append(cs)
+    dr = DeviceReplicas([s.chain for s in systems], [s.field for s in systems], ctrls,
+                        rng_mode=L.RNG_PHILOX, seeds=[20240601] * 512, streams=list(range(512)),
+                        log_cap=161 * sweeps + 16)
+    picks = (0, 101, 255, 388, 511)
+    E0 = {i: _total_energy(systems[i].chain, systems[i]) for i in picks}
+    say("run", sweeps, "sweeps")
+    dr.run(sweeps, waves=2)
+    dr.raise_on_error()
+    inc = {}
+    for i in picks:
+        lg = dr.logs(i)
+        acc = lg["accepted"] != 0
+        dE = float(np.sum(lg["dE_poly"][acc]) + np.sum(lg["dE_field"][acc]))
+        E1 = _total_energy(_final_chain(dr, i, systems[i].chain), systems[i])
+        assert abs(E0[i]) < 1e90 and abs(E1) < 1e90
+        say(f"  replica {i}: N {systems[i].chain.N} accepted {int(acc.sum())} "
+            f"E1 - E0 - sum dE = {(E1 - E0[i]) - dE:.3e}")
+        assert abs((E1 - E0[i]) - dE) < 1e-12 * (abs(E0[i]) + abs(E1)) + 1e-7
+        inc[i] = _check_state(dr, i, systems[i])
+    dr.density_init()
+    for i in picks:
+        f = systems[i].field
+        one_bead = 1.0 / (f.x_width * f.y_width * f.z_width / (f.nx * f.ny * f.nz))
+        assert np.max(np.abs(inc[i] - dr.density_of(i))) < 1e-9 * one_bead
