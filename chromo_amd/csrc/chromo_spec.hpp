@@ -53,7 +53,9 @@ struct SpecRes {
 struct SpecCtl {
     int32_t nq, cnt, remaining, err;
     int32_t used;                             // conflict-table insertions this round
-    int32_t pad[3];
+    int32_t head;                             // ring position of the round's first slot
+    int32_t tag;                              // the round's number in the launch (never repeats)
+    int32_t ready;                            // = tag once the later slots' conflict masks are set
 };
 
 // LDS exchange area of the one-at-a-time path (seq_step).
@@ -602,7 +604,14 @@ __device__ __forceinline__ bool eval_slot(const chromo_replica *R, const SpecSlo
 // Earlier slots of this round whose writes slot s reads: shared voxels through
 // the conflict table, shared beads through the bead ranges (tangent rotation:
 // picks within one bead of each other).
-__device__ __forceinline__ void spec_conflicts(const SpecSlot *slots, SpecRes *res, const SpecLds &L,
+template <int RING>
+__device__ __forceinline__ const SpecSlot &ring_at(const SpecSlot *ring, int head, int s)
+{
+    return ring[(head + s) & (RING - 1)];
+}
+
+template <int RING>
+__device__ __forceinline__ void spec_conflicts(const SpecSlot *ring, int head, SpecRes *res, const SpecLds &L,
                                                const SegLds &G, int s)
 {
     static_assert((SPEC_WMAX - 1) * SPEC_NMAX <= WAVE, "one lane per (earlier slot, pick)");
@@ -614,17 +623,18 @@ __device__ __forceinline__ void spec_conflicts(const SpecSlot *slots, SpecRes *r
         const int h = G.uaux[q];
         if (h >= 0) m |= L.hmask[h] & low;
     }
-    const SpecSlot &S = slots[s];
+    const SpecSlot &S = ring_at<RING>(ring, head, s);
     if (S.move == CHROMO_TANGENT_ROTATION) {
         const int a = lane / SPEC_NMAX, c = lane % SPEC_NMAX;
-        if (a < s && slots[a].kind == SLOT_SMALL && c < S.n) {
+        if (a < s && ring_at<RING>(ring, head, a).kind == SLOT_SMALL && c < S.n) {
+            const SpecSlot &A = ring_at<RING>(ring, head, a);
             const int64_t mine = S.picks[c];
-            for (int64_t q = 0; q < slots[a].n; ++q) {
-                const int64_t d = slots[a].picks[q] - mine;
+            for (int64_t q = 0; q < A.n; ++q) {
+                const int64_t d = A.picks[q] - mine;
                 if (d >= -1 && d <= 1) m |= 1u << a;
             }
         }
-    } else if (lane < s && slots[lane].kind == SLOT_SMALL) {
+    } else if (lane < s && ring_at<RING>(ring, head, lane).kind == SLOT_SMALL) {
         const SpecRes &o = res[lane];
         if (o.wlo <= res[s].rhi && res[s].rlo < o.whi) m |= 1u << lane;
     }
@@ -645,66 +655,35 @@ __device__ __forceinline__ void spec_release(const SpecLds &L, const SegLds &G, 
     }
 }
 
-__device__ __forceinline__ void slot_copy(SpecSlot &dst, const SpecSlot &src)
+// Wavefront 0: appends proposals to the queue of `q` slots starting at ring
+// position head until it holds `limit`, or `remaining` (the block's unresolved
+// proposals), or ends with a SLOT_SEQ one.  Returns the new queue length.
+template <int NB, int RING>
+__device__ __forceinline__ int spec_gen(const chromo_replica *R, WaveRng &rng, const chromo_ctrl &C, SpecSlot *ring,
+                                        int head, int q, int limit, int remaining)
 {
-    const uint32_t *s = (const uint32_t *)&src;
-    uint32_t *d = (uint32_t *)&dst;
-    for (int i = lane_id(); i < (int)(sizeof(SpecSlot) / 4); i += WAVE) d[i] = s[i];
-}
-
-
-// Wavefront 0: copies the unresolved slots first..nq-1 of cs to the front of ns
-// (only slot `first`, marked SLOT_SEQ, when rewind is set).  Returns their count.
-__device__ __forceinline__ int spec_carry(const SpecSlot *cs, SpecSlot *ns, int first, int nq, int rewind)
-{
-    int j = 0;
-    if (rewind) {
-        slot_copy(ns[0], cs[first]);
-        wave_sync();
-        if (lane_id() == 0) ns[0].kind = SLOT_SEQ;
-        j = 1;
-    } else {
-        for (int s = first; s < nq; ++s) slot_copy(ns[j++], cs[s]);
-    }
-    wave_sync();
-    return j;
-}
-
-// Wavefront 0: tops the queue cs[0..nq) up with new proposals -- at most W, at
-// most `remaining`, none after a SLOT_SEQ one -- and publishes the control words.
-template <int NB, int W>
-__device__ __forceinline__ void spec_fill(const chromo_replica *R, WaveRng &rng, const chromo_ctrl &C,
-                                          const SpecLds &L, SpecSlot *cs, int nq, int remaining)
-{
-    int j = nq;
-    bool stop = j > 0 && cs[j - 1].kind == SLOT_SEQ;
-    while (!stop && j < W && j < remaining) {
-        const int kind = gen_slot<NB>(R, rng, C, cs[j]);
-        ++j;
+    bool stop = q > 0 && ring_at<RING>(ring, head, q - 1).kind == SLOT_SEQ;
+    while (!stop && q < limit && q < remaining) {
+        const int kind = gen_slot<NB>(R, rng, C, ring[(head + q) & (RING - 1)]);
+        ++q;
         stop = kind == SLOT_SEQ;
     }
-    int cnt = 0;
-    while (cnt < j && cs[cnt].kind != SLOT_SEQ) ++cnt;
-    if (lane_id() == 0) {
-        L.ctl->nq = j;
-        L.ctl->cnt = cnt;
-        L.ctl->remaining = remaining;
-        L.ctl->used = 0;
-    }
+    return q;
 }
 
 // One wavefront: AcceptanceTracker updates (mc_stat.py:190-208), controller counts,
 // replica counters and log records of the resolved slots 0..F-1, one slot per
 // lane; the same totals as record() called once per slot in order.
+template <int RING>
 __device__ __forceinline__ void spec_book(const chromo_replica *R, chromo_ctrl &C, Counters &K,
-                                          const SpecSlot *cs, const SpecRes *res, int F)
+                                          const SpecSlot *ring, int head, const SpecRes *res, int F)
 {
     const int lane = lane_id();
-    const bool live = lane < F && cs[lane < F ? lane : 0].kind != SLOT_EMPTY;
+    const bool live = lane < F && ring_at<RING>(ring, head, lane < F ? lane : 0).kind != SLOT_EMPTY;
     int64_t n = 0, nb = 0, bytes = 0;
     int acc = 0, tie = 0;
     if (live) {
-        const SpecSlot &S = cs[lane];
+        const SpecSlot &S = ring_at<RING>(ring, head, lane);
         const SpecRes &q = res[lane];
         n = S.n;
         nb = q.nbins;
@@ -722,7 +701,7 @@ __device__ __forceinline__ void spec_book(const chromo_replica *R, chromo_ctrl &
     if (R->log != nullptr && live) {
         const int64_t at = logged + __popcll(live_m & ((1ull << lane) - 1));
         if (at < R->log_cap) {
-            const SpecSlot &S = cs[lane];
+            const SpecSlot &S = ring_at<RING>(ring, head, lane);
             const SpecRes &q = res[lane];
             chromo_move_log &Lg = R->log[at];
             Lg.move = S.move;
@@ -765,35 +744,60 @@ __device__ __forceinline__ void spec_book(const chromo_replica *R, chromo_ctrl &
 
 // One controller block.  C and K live in LDS: only wavefront 0 touches them, and
 // keeping them out of registers leaves those to the evaluation.
+//
+// The slots wait in a ring of 2W.  A round evaluates the first min(W, queued)
+// of them.  Slot 0 of a round stands whatever the others do (nothing precedes
+// it), so wavefront 0 commits it as soon as it is evaluated -- a later slot that
+// reads what it writes is redone anyway -- and then draws the following
+// proposals into the ring while the other wavefronts finish.
 template <int NB, int W>
 __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng, chromo_ctrl &C, Counters &K,
                                            const SpecLds &L)
 {
+    constexpr int RING = 2 * W;
+    static_assert((RING & (RING - 1)) == 0, "ring positions wrap by mask");
     const int wv = wave_id(), lane = lane_id();
-    int cur = 0;
-    int nq = 0;                        // wavefront 0: slots queued in the current buffer
+    SpecSlot *ring = L.slots[0];
+    int head = 0;                      // wavefront 0: ring position of the first queued slot
+    int nq = 0;                        // wavefront 0: slots queued
     int remaining = C.num_per_cycle;   // wavefront 0: proposals of the block not yet resolved
     int pend = 0;                      // wavefront 0: error to publish before the next barrier
+    int32_t tag = 0;                   // wavefront 0: the round's number in the launch
     int64_t rounds = 0;
     // every round resolves a proposal or turns its first slot into a sequential one
     const int64_t max_rounds = 2 * (int64_t)C.num_per_cycle + 64;
+    // drawing ahead is off where a redone proposal would have to rewind MT19937
+    const bool ahead = !(rng.mode == CHROMO_RNG_GLIBC && C.move == CHROMO_CHANGE_BINDING_STATE);
     for (;;) {
-        SpecSlot *cs = cur ? L.slots[1] : L.slots[0];
-        SpecSlot *ns = cur ? L.slots[0] : L.slots[1];
         if (wv == 0) {
             // L.ctl->err is written only here, before the barrier after which every
             // wavefront reads it
             if (++rounds > max_rounds) pend = ERR_SPIN;
             if (pend && lane == 0) L.ctl->err = pend;
-            if (!pend && !L.ctl->err) spec_fill<NB, W>(R, rng, C, L, cs, nq, remaining);
+            if (!pend && !L.ctl->err) {
+                tag = uni_i32(L.ctl->tag) + 1;
+                nq = spec_gen<NB, RING>(R, rng, C, ring, head, nq, W, remaining);
+                const int nr = nq < W ? nq : W;
+                int cnt = 0;
+                while (cnt < nr && ring_at<RING>(ring, head, cnt).kind != SLOT_SEQ) ++cnt;
+                if (lane == 0) {
+                    L.ctl->nq = nr;
+                    L.ctl->cnt = cnt;
+                    L.ctl->remaining = remaining;
+                    L.ctl->used = 0;
+                    L.ctl->head = head;
+                    L.ctl->tag = tag;
+                }
+            }
         }
         __syncthreads();
         if (L.ctl->err || L.ctl->remaining <= 0) break;
-        const int cnt = L.ctl->cnt, nqa = L.ctl->nq;
-        if (cs[0].kind == SLOT_SEQ) {
+        const int cnt = L.ctl->cnt, h = L.ctl->head;
+        if (ring_at<RING>(ring, h, 0).kind == SLOT_SEQ) {
             if (wv == 0) {
                 SeqIO *io = L.seq;
-                for (int i = lane; i < CK_WORDS; i += WAVE) io->ck[i] = cs[0].ck[i];
+                const SpecSlot &S0 = ring_at<RING>(ring, h, 0);
+                for (int i = lane; i < CK_WORDS; i += WAVE) io->ck[i] = S0.ck[i];
                 if (lane == 0) {
                     io->mti = rng.mti;
                     io->C = C;
@@ -813,22 +817,52 @@ __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng
                     record(R, C, o, K);
                 }
                 remaining -= 1;
+                // nothing is drawn past a SLOT_SEQ slot
+                head = (head + 1) & (RING - 1);
                 nq = 0;
             }
         } else {
             SegKeep<NB> keep;
-            const bool mine = wv < cnt && cs[wv].kind == SLOT_SMALL;
+            const SpecSlot &mys = ring_at<RING>(ring, h, wv < W ? wv : 0);
+            const bool mine = wv < cnt && mys.kind == SLOT_SMALL;
             bool inserted = false;
-            if (mine) inserted = eval_slot<NB>(R, cs[wv], L.res[wv], keep, L, L.mine, wv);
+            if (mine) inserted = eval_slot<NB>(R, mys, L.res[wv], keep, L, L.mine, wv);
+            bool committed = false;
+            if (wv == 0 && mine && L.res[0].accepted && !L.res[0].resample && !L.res[0].overflow) {
+                const int mynew = lane < mys.n ? mys.newst[lane] : 0;
+                seg_commit<NB>(R, mys.move, mys.n, mys.binder, mynew, keep, L.mine, L.res[0].nbins);
+                committed = true;
+            }
             __syncthreads();
-            if (mine && !L.res[wv].resample && !L.res[wv].overflow) spec_conflicts(cs, L.res, L, L.mine, wv);
-            __syncthreads();
+            if (wv > 0) {
+                // the later slots' conflict masks, then the flag wavefront 0 waits on
+                if (mine && !L.res[wv].resample && !L.res[wv].overflow)
+                    spec_conflicts<RING>(ring, h, L.res, L, L.mine, wv);
+                static_assert(W <= 2, "one wavefront sets the ready flag");
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                if (lane == 0) __hip_atomic_store(&L.ctl->ready, L.ctl->tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else {
+                // draw the following proposals while the other wavefronts check
+                // conflicts, commit and book
+                if (ahead && !pend) nq = spec_gen<NB, RING>(R, rng, C, ring, head, nq, RING, remaining);
+                if (W > 1) {
+                    int spins = 0;
+                    while (uni_i32(__hip_atomic_load(&L.ctl->ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) !=
+                           tag) {
+                        if (++spins >= SPIN_LIMIT) {
+                            pend = ERR_SPIN;
+                            break;
+                        }
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                }
+            }
             // resolution, identical in every wavefront; lane s reads slot s's result,
             // all slots in one LDS round trip
             uint32_t kd = SLOT_EMPTY, cmk = 0, ov = 0, rs = 0, ac = 0;
             if (lane < cnt) {
                 const SpecRes &q = L.res[lane];
-                kd = (uint32_t)cs[lane].kind;
+                kd = (uint32_t)ring_at<RING>(ring, h, lane).kind;
                 cmk = q.cmask;
                 ov = (uint32_t)q.overflow;
                 rs = (uint32_t)q.resample;
@@ -852,23 +886,27 @@ __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng
                 }
                 if (readlane_u32(ac, s)) accm |= 1u << s;
             }
-            if (mine && wv < F && L.res[wv].accepted) {
-                const SpecSlot &S = cs[wv];
-                const int mynew = lane < S.n ? S.newst[lane] : 0;
-                seg_commit<NB>(R, S.move, S.n, S.binder, mynew, keep, L.mine, L.res[wv].nbins);
+            if (mine && !committed && wv < F && L.res[wv].accepted) {
+                const int mynew = lane < mys.n ? mys.newst[lane] : 0;
+                seg_commit<NB>(R, mys.move, mys.n, mys.binder, mynew, keep, L.mine, L.res[wv].nbins);
             }
             if (inserted) spec_release(L, L.mine, L.res[wv].nbins);
-            // the last wavefront books the round while wavefront 0 carries the
-            // unresolved slots; only the barrier below orders C and K for the
-            // next reader
-            if (wv == W - 1) spec_book(R, C, K, cs, L.res, F);
+            // the last wavefront books the round; only the barrier below orders C
+            // and K for the next reader
+            if (wv == W - 1) spec_book<RING>(R, C, K, ring, h, L.res, F);
             if (wv == 0) {
-                nq = spec_carry(cs, ns, F, nqa, rew);
+                if (rew) {
+                    // slot F is redone one at a time; what was drawn after it is
+                    // dropped (the sequential path rewinds the streams to it)
+                    if (lane == 0) ring[(head + F) & (RING - 1)].kind = SLOT_SEQ;
+                    nq = F + 1;
+                }
+                head = (head + F) & (RING - 1);
+                nq -= F;
                 remaining -= F;
             }
         }
         __syncthreads();
-        cur ^= 1;
     }
 }
 
@@ -907,6 +945,8 @@ __device__ __forceinline__ void spec_run(const chromo_replica *R, int64_t steps)
             counters_load(R, sK);
             for (int i = 0; i < nc; ++i) sctrl[i] = R->ctrl[i];
             L.ctl->err = 0;
+            L.ctl->tag = 0;
+            L.ctl->ready = -1;
         }
     }
     __syncthreads();
