@@ -56,6 +56,7 @@ struct SpecCtl {
     int32_t head;                             // ring position of the round's first slot
     int32_t tag;                              // the round's number in the launch (never repeats)
     int32_t ready;                            // = tag once the later slots' conflict masks are set
+    int32_t ins;                              // = tag once slot 0's voxels are in the conflict table
 };
 
 // LDS exchange area of the one-at-a-time path (seq_step).
@@ -828,19 +829,33 @@ __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng
             bool inserted = false;
             if (mine) inserted = eval_slot<NB>(R, mys, L.res[wv], keep, L, L.mine, wv);
             bool committed = false;
-            if (wv == 0 && mine && L.res[0].accepted && !L.res[0].resample && !L.res[0].overflow) {
-                const int mynew = lane < mys.n ? mys.newst[lane] : 0;
-                seg_commit<NB>(R, mys.move, mys.n, mys.binder, mynew, keep, L.mine, L.res[0].nbins);
-                committed = true;
+            static_assert(W <= 2, "one wavefront on each side of the round's flags");
+            if (wv == 0) {
+                // slot 0's voxels are in the conflict table: the later slot may check
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                if (lane == 0) __hip_atomic_store(&L.ctl->ins, L.ctl->tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (mine && L.res[0].accepted && !L.res[0].resample && !L.res[0].overflow) {
+                    const int mynew = lane < mys.n ? mys.newst[lane] : 0;
+                    seg_commit<NB>(R, mys.move, mys.n, mys.binder, mynew, keep, L.mine, L.res[0].nbins);
+                    committed = true;
+                }
             }
-            __syncthreads();
             if (wv > 0) {
-                // the later slots' conflict masks, then the flag wavefront 0 waits on
+                // wait for slot 0's table entries, set the later slot's conflict mask,
+                // then the flag wavefront 0 waits on
+                const int32_t tag = uni_i32(L.ctl->tag);
+                int spins = 0;
+                while (uni_i32(__hip_atomic_load(&L.ctl->ins, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) != tag) {
+                    if (++spins >= SPIN_LIMIT) break;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                // a table that did not arrive: the slot waits for the next round
+                if (spins >= SPIN_LIMIT && mine && lane == 0) L.res[wv].overflow = 1;
+                wave_sync();
                 if (mine && !L.res[wv].resample && !L.res[wv].overflow)
                     spec_conflicts<RING>(ring, h, L.res, L, L.mine, wv);
-                static_assert(W <= 2, "one wavefront sets the ready flag");
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                if (lane == 0) __hip_atomic_store(&L.ctl->ready, L.ctl->tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (lane == 0) __hip_atomic_store(&L.ctl->ready, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             } else {
                 // draw the following proposals while the other wavefronts check
                 // conflicts, commit and book
@@ -947,6 +962,7 @@ __device__ __forceinline__ void spec_run(const chromo_replica *R, int64_t steps)
             L.ctl->err = 0;
             L.ctl->tag = 0;
             L.ctl->ready = -1;
+            L.ctl->ins = -1;
         }
     }
     __syncthreads();
