@@ -338,6 +338,16 @@ __device__ __forceinline__ void philox_get(const WaveRng &R, uint64_t &ctr, int 
     }
 }
 
+// The position (ctr, bi) of philox_get without the block words: all philox_set
+// needs.
+__device__ __forceinline__ void philox_pos(const WaveRng &R, uint64_t &ctr, int &bi)
+{
+    const uint64_t blk = R.pbase + (uint64_t)(R.pidx >> 2);
+    const bool start = (R.pidx & 3) == 0;
+    ctr = start ? blk : blk + 1;
+    bi = start ? 4 : (R.pidx & 3);
+}
+
 // Moves to the position (ctr, bi) of philox_get; recomputes the LDS batch only
 // when that position lies outside the batch it holds.
 __device__ __forceinline__ void philox_set(WaveRng &R, uint64_t ctr, int bi)

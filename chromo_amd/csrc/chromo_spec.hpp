@@ -110,23 +110,21 @@ struct SpecLds {
     SegLds first;   // wavefront 0's (the one-at-a-time path)
     char *wide;     // the W blocks as one wide block (wide_lds_bytes), or nullptr if they are too small
 };
+// A stream checkpoint in LDS: the glibc words and indices, and the Philox position
+// (counter, index in block; words 35-38 are not used: rng_restore needs the
+// position only).
 __device__ __forceinline__ void rng_save(const WaveRng &g, uint32_t *ck)
 {
     const int lane = lane_id();
     uint64_t ctr = 0;
     int bi = 4;
-    uint32_t b[4] = {0, 0, 0, 0};
-    if (g.mode == CHROMO_RNG_PHILOX) philox_get(g, ctr, bi, b);
+    if (g.mode == CHROMO_RNG_PHILOX) philox_pos(g, ctr, bi);
     if (lane < 31) ck[lane] = g.g;
     if (lane == 0) {
         ck[31] = (uint32_t)g.gf;
         ck[32] = (uint32_t)g.gr;
         ck[33] = (uint32_t)ctr;
         ck[34] = (uint32_t)(ctr >> 32);
-        ck[35] = b[0];
-        ck[36] = b[1];
-        ck[37] = b[2];
-        ck[38] = b[3];
         ck[39] = (uint32_t)bi;
     }
     wave_sync();
