@@ -661,6 +661,7 @@ template <int NB, int RING>
 __device__ __forceinline__ int spec_gen(const chromo_replica *R, WaveRng &rng, const chromo_ctrl &C, SpecSlot *ring,
                                         int head, int q, int limit, int remaining)
 {
+    if (q >= limit || q >= remaining) return q;
     bool stop = q > 0 && ring_at<RING>(ring, head, q - 1).kind == SLOT_SEQ;
     while (!stop && q < limit && q < remaining) {
         const int kind = gen_slot<NB>(R, rng, C, ring[(head + q) & (RING - 1)]);
@@ -761,7 +762,7 @@ __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng
     int nq = 0;                        // wavefront 0: slots queued
     int remaining = C.num_per_cycle;   // wavefront 0: proposals of the block not yet resolved
     int pend = 0;                      // wavefront 0: error to publish before the next barrier
-    int32_t tag = 0;                   // wavefront 0: the round's number in the launch
+    int32_t tag = uni_i32(L.ctl->tag); // wavefront 0: the round's number in the launch
     int64_t rounds = 0;
     // every round resolves a proposal or turns its first slot into a sequential one
     const int64_t max_rounds = 2 * (int64_t)C.num_per_cycle + 64;
@@ -773,12 +774,15 @@ __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng
             // wavefront reads it
             if (++rounds > max_rounds) pend = ERR_SPIN;
             if (pend && lane == 0) L.ctl->err = pend;
-            if (!pend && !L.ctl->err) {
-                tag = uni_i32(L.ctl->tag) + 1;
+            // only wavefront 0 writes L.ctl->err and L.ctl->tag: pend and tag are their values
+            if (!pend) {
+                tag += 1;
                 nq = spec_gen<NB, RING>(R, rng, C, ring, head, nq, W, remaining);
                 const int nr = nq < W ? nq : W;
-                int cnt = 0;
-                while (cnt < nr && ring_at<RING>(ring, head, cnt).kind != SLOT_SEQ) ++cnt;
+                // leading slots that are not SLOT_SEQ, all kinds read at once
+                const int kd = lane < nr ? ring_at<RING>(ring, head, lane).kind : SLOT_SEQ;
+                const uint64_t seqm = __ballot(kd == SLOT_SEQ);
+                const int cnt = seqm ? (int)__builtin_ctzll(seqm) : nr;
                 if (lane == 0) {
                     L.ctl->nq = nr;
                     L.ctl->cnt = cnt;
