@@ -187,11 +187,35 @@ __device__ __forceinline__ void seg_transform(const Proposal &P, const double o[
 
 __device__ __forceinline__ int wrap1(int c, int nn) { return c + 1 == nn ? 0 : c + 1; }
 
+// Neighbour block of a crank-shaft, end-pivot or slide move, one per wavefront
+// (at most two per workgroup), filled straight from HBM into LDS: as doubles,
+// 0-2 r and 3-5 t3 of bead ind0 - 1, 6-10 bond (ind0 - 1, ind0), 11-13 r and
+// 14-16 t3 of bead indf, 17-21 bond (indf - 1, indf).
+constexpr int SEG_NBR_WAVES = 2;
+constexpr int SEG_NBR_DOUBLES = 22;
+
+__device__ __forceinline__ double *seg_nbr()
+{
+    __shared__ double nbr[SEG_NBR_WAVES][32];
+    return nbr[wave_id()];
+}
+
+// The neighbour block once its direct-to-LDS loads have landed.  The compiler's
+// wait insertion does not tie these loads to later LDS reads of the block (it
+// places its vmcnt wait after them), so the wait is explicit, and the asm's
+// memory clobber keeps the reads below it.
+__device__ __forceinline__ const double *seg_nbr_ready()
+{
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    return seg_nbr();
+}
+
 // What an evaluation reads of the chain, loaded in one batch before any of its
 // arithmetic: lane i's bead (window bead ind0 + i, or tangent pick i), its
 // coordinates, binder states and (binding moves) modifications, and the r, t3
 // and bond parameters of the neighbours the polymer dE needs: beads ind0 - 1 and
-// indf on lanes 0 and n - 1, beads b - 1 and b + 1 of every tangent pick.
+// indf in the wavefront's neighbour block (seg_nbr), beads b - 1 and b + 1 of
+// every tangent pick in registers.
 template <int NB>
 struct SegBeads {
     int64_t b;
@@ -235,21 +259,37 @@ __device__ __forceinline__ void seg_load(const chromo_replica *R, int mv, int64_
     }
     B.hasl = act && !binding && (tangent ? b != 0 : (lane == 0 && ind0 != 0));
     B.hasr = act && !binding && (tangent ? b + 1 != N : (lane == n - 1 && indf != N));
-    if (B.hasl) {
+    if (tangent) {
+        if (B.hasl) {
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            B.lr[j] = Pp[j * N + b - 1];
-            B.lr[3 + j] = Pp[(3 + j) * N + b - 1];
+            for (int j = 0; j < 3; ++j) {
+                B.lr[j] = Pp[j * N + b - 1];
+                B.lr[3 + j] = Pp[(3 + j) * N + b - 1];
+            }
+            B.lb = load_bond(R->bonds, N, b - 1);
         }
-        B.lb = load_bond(R->bonds, N, b - 1);
-    }
-    if (B.hasr) {
+        if (B.hasr) {
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            B.rr[j] = Pp[j * N + b + 1];
-            B.rr[3 + j] = Pp[(3 + j) * N + b + 1];
+            for (int j = 0; j < 3; ++j) {
+                B.rr[j] = Pp[j * N + b + 1];
+                B.rr[3 + j] = Pp[(3 + j) * N + b + 1];
+            }
+            B.rb = load_bond(R->bonds, N, b);
         }
-        B.rb = load_bond(R->bonds, N, b);
+    } else if (!binding && N >= 2) {
+        // dword j of the neighbour block on lane j, loaded into LDS directly: no
+        // registers held while the proposal arithmetic runs
+        const int d = lane >> 1;
+        if (d < SEG_NBR_DOUBLES) {
+            const bool left = d < 11;
+            const int q = left ? d : d - 11;
+            int64_t bead = left ? ind0 - 1 : indf, bond = left ? ind0 - 1 : indf - 1;
+            bead = bead < 0 ? 0 : (bead > N - 1 ? N - 1 : bead);
+            bond = bond < 0 ? 0 : (bond > N - 2 ? N - 2 : bond);
+            const double *src = q < 6 ? R->pos + (int64_t)q * N + bead : R->bonds + (int64_t)(q - 6) * (N - 1) + bond;
+            __builtin_amdgcn_global_load_lds((const void *)((const uint32_t *)src + (lane & 1)),
+                                             (void __attribute__((address_space(3))) *)seg_nbr(), 4, 0, 0);
+        }
     }
 }
 
@@ -260,9 +300,11 @@ __device__ __forceinline__ void seg_r3(const SegBeads<NB> &B, int64_t x, int64_t
                                        double v[3])
 {
     if (x == ind0 - 1) {
-        for (int j = 0; j < 3; ++j) v[j] = readlane_f64(B.lr[j], 0);
+        const double *nb = seg_nbr_ready();
+        for (int j = 0; j < 3; ++j) v[j] = nb[j];
     } else if (x == indf) {
-        for (int j = 0; j < 3; ++j) v[j] = readlane_f64(B.rr[j], (int)(n - 1));
+        const double *nb = seg_nbr_ready();
+        for (int j = 0; j < 3; ++j) v[j] = nb[11 + j];
     } else {
         for (int j = 0; j < 3; ++j) v[j] = readlane_f64(B.o[j], (int)(x - ind0));
     }
@@ -322,9 +364,23 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
         }
         for (int i = 0; i < n; ++i) dEp += readlane_f64(e, i);
     } else {
+        // the neighbours' r, t3 and bonds from the wavefront's neighbour block
+        const double *nb = seg_nbr_ready();
         double ef = 0, er = 0;
-        if (lane == 0 && ind0 != 0) ef = pair_dE_forward(B.lb, &B.lr[0], &o[0], &w[0], &B.lr[3], &o[3], &w[3]);
-        if (lane == n - 1 && indf != N) er = pair_dE_reverse(B.rb, &o[0], &w[0], &B.rr[0], &o[3], &w[3], &B.rr[3]);
+        if (lane == 0 && ind0 != 0) {
+            double r0[6];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) r0[j] = nb[j];
+            const Bond bd{nb[6], nb[7], nb[8], nb[9], nb[10]};
+            ef = pair_dE_forward(bd, &r0[0], &o[0], &w[0], &r0[3], &o[3], &w[3]);
+        }
+        if (lane == n - 1 && indf != N) {
+            double r1[6];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) r1[j] = nb[11 + j];
+            const Bond bd{nb[17], nb[18], nb[19], nb[20], nb[21]};
+            er = pair_dE_reverse(bd, &o[0], &w[0], &r1[0], &o[3], &w[3], &r1[3]);
+        }
         if (ind0 != 0) dEp += readlane_f64(ef, 0);
         if (indf != N) dEp += readlane_f64(er, (int)(n - 1));
     }
