@@ -167,6 +167,21 @@ def test_mc_sim_glibc_parity(oracle, N, nvox, seed, waves):
     assert lo["accepted"].sum() > 100
 
 
+@pytest.mark.parametrize("waves", [1, 2])
+@pytest.mark.parametrize("rng_mode", [0, 1], ids=["glibc", "philox"])
+@pytest.mark.parametrize("per_cycle", [(120, 0, 0, 0, 0), (0, 0, 120, 0, 0), (0, 40, 0, 0, 0)],
+                         ids=["crank_shaft", "slide", "end_pivot"])
+def test_single_move_chains(oracle, per_cycle, rng_mode, waves):
+    """One move type per chain.  Every proposal reads the window's two neighbour
+    beads and bonds (the crank-shaft axis, the end-pivot fulcrum, the polymer dE
+    at the window's ends), so a neighbour read that misses its load shows in the
+    first few proposals."""
+    say("single move", per_cycle, rng_mode, waves)
+    s = W.make_system(4000, 2, 20, seed=17, per_cycle=per_cycle)
+    lo = _compare_run(oracle, s, steps=12, rng_mode=rng_mode, seed=99, stream=3, waves=waves)
+    assert lo["accepted"].sum() > 10
+
+
 @pytest.mark.parametrize("waves", [1, 2, pytest.param(8, marks=pytest.mark.skip(reason=W8_SKIP))])
 def test_mc_sim_philox_parity(oracle, waves):
     say("mc_sim philox")
