@@ -14,6 +14,9 @@ Also reported: the plain path (one kernel launch per proposal) on the
 of four replicas against the CPU oracle, the roofline of the dominant kernel
 (algorithmic HBM bytes per launch over its HIP-event-timed duration), and the
 CPU oracle timed on a bounded sample of the same replicas on the host cores.
+
+``--adjoint`` instead times the energy + adjoint evaluation (E, dE/dr, dE/dtheta)
+on the 20,000-bead and 10^6-bead configurations and prints its own JSON line.
 """
 from __future__ import annotations
 
@@ -154,6 +157,39 @@ def cpu_baseline(systems, target_s=15.0):
                        f"{p} proposals in {sec:.1f} s, oracle lean build")
 
 
+def bench_adjoint(args):
+    """Energy + adjoint (E, dE/dr, dE/dtheta through chromo_energy_adjoint) per second
+    on the 20,000-bead / B = 3 / 48^3 configuration and on one 10^6-bead / B = 4 /
+    128^3 chain, inputs resident in HBM.  Algorithmic bytes per evaluation:
+    N (48 + 2B) (positions, states, marks) + 2 * 8 (B+1) V (density write and read)
+    + 24 N (gradient write)."""
+    import torch
+    from chromo_amd import workloads as W
+    from chromo_amd.energy import EnergyModel, energy
+    out = []
+    for name, s in (("20,000 beads, B=3, 48^3 voxels", W.config_adjoint()),
+                    ("1,000,000 beads, B=4, 128^3 voxels", W.config_large(1)[0])):
+        model = EnergyModel(s.chain, s.field, s.binders)
+        r = torch.tensor(s.chain.r, dtype=torch.float64, device="cuda")
+        th = torch.tensor(model.theta0, dtype=torch.float64, device="cuda")
+        for _ in range(max(1, args.warmup)):
+            energy(model, r, th, grad=True)
+        k = args.adjoint_steps
+        secs = timed(lambda: energy(model, r, th, grad=True), k)
+        N, B = s.chain.N, s.chain.B
+        V = s.field.nx * s.field.ny * s.field.nz
+        nbytes = N * (48 + 2 * B) + 2 * 8 * (B + 1) * V + 24 * N
+        gbs = nbytes * k / secs / 1e9
+        out.append({"workload": name, "N": N, "B": B, "voxels": V, "evaluations": k,
+                    "ms_per_evaluation": 1e3 * secs / k, "evaluations_per_s": k / secs,
+                    "algorithmic_bytes_per_evaluation": nbytes, "achieved_GBs": gbs,
+                    "frac_of_hbm_peak": gbs / HBM_PEAK_GBS})
+        log(f"adjoint {name}: {1e3 * secs / k:.4f} ms per evaluation, {gbs:.1f} GB/s")
+        del model, r, th
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +206,9 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 256 MiB device to device 5 times (calibrates rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE counters against a known byte count)")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="only time the energy + adjoint evaluation (its own JSON line)")
+    ap.add_argument("--adjoint-steps", type=int, default=50)
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -189,6 +228,12 @@ def main():
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "needs a ROCm GPU"
+    if args.adjoint:
+        print(json.dumps({"metric": "energy + adjoint evaluations/s (E, dE/dr, dE/dtheta), FP64",
+                          "unit": "evaluations/s", "n_gpus": 1, "dtype": "f64", "data": "synthetic",
+                          "higher_is_better": True, "hbm_peak_GBs": HBM_PEAK_GBS,
+                          "workloads": bench_adjoint(args)}), flush=True)
+        return
 
     b = bench_batched(args)
     value = b["proposals"] / b["secs"]

@@ -119,63 +119,96 @@ __global__ void k_dens_sort(int64_t nbins, const int32_t *off, const int32_t *cn
     }
 }
 
-// density[v * Bp1 + m]: m = 0 beads, m >= 1 binder m-1 (weight x state)
+// density[v * Bp1 + m]: m = 0 beads, m >= 1 binder m-1 (weight x state).  Voxel v
+// is corner l of the beads whose lower-corner cell is v - (l & 1, l >> 1 & 1,
+// l >> 2 & 1), wrapped.  Those <= 8 sorted buckets are merged in ascending bead
+// index, and the corners of one bead that land on v (several only on grids one or
+// two voxels wide, where buckets coincide) are folded in corner order: the
+// reference's bead loop.  Every per-bucket value is indexed at compile time, so
+// the merge state stays in registers.
 __global__ void k_dens_gather(GridGeom G, int64_t N, int Bp1, const int8_t *states,
                               const int32_t *off, const int32_t *cnt, const int32_t *list,
                               const double *wts, double *density, int cleanup)
 {
+    constexpr int32_t NONE = 0x7fffffff;
     int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nx = G.nx, ny = G.ny, nz = G.nz;
     if (v >= nx * ny * nz) return;
-    int64_t x = v % nx, y = (v / nx) % ny, z = v / (nx * ny);
-    int32_t cells[8];
-    int masks[8];
-    int nc = 0;
+    const int64_t x = v % nx, y = (v / nx) % ny, z = v / (nx * ny);
+    int32_t cell[8], pos[8], end[8], head[8];
+    int msk[8];
+#pragma unroll
     for (int l = 0; l < 8; ++l) {
         int64_t cx = x - (l & 1), cy = y - ((l >> 1) & 1), cz = z - ((l >> 2) & 1);
         if (cx < 0) cx += nx;
         if (cy < 0) cy += ny;
         if (cz < 0) cz += nz;
-        int32_t c = (int32_t)(cx + cy * nx + cz * nx * ny);
-        int f = -1;
-        for (int q = 0; q < nc; ++q)
-            if (cells[q] == c) f = q;
-        if (f < 0) {
-            cells[nc] = c;
-            masks[nc] = 0;
-            f = nc++;
-        }
-        masks[f] |= 1 << l;
+        cell[l] = (int32_t)(cx + cy * nx + cz * nx * ny);
+        msk[l] = 0;
     }
-    int32_t pos[8], end[8];
-    for (int q = 0; q < nc; ++q) {
-        pos[q] = off[cells[q]];
-        end[q] = pos[q] + cnt[cells[q]];
-    }
-    double acc[CHROMO_MAX_BINDERS + 1];
-    for (int m = 0; m < Bp1; ++m) acc[m] = 0;
-    for (;;) {
-        int best = -1;
-        int32_t bb = 0x7fffffff;
-        for (int q = 0; q < nc; ++q)
-            if (pos[q] < end[q] && list[pos[q]] < bb) {
-                bb = list[pos[q]];
-                best = q;
+    // a bucket reached by several corners is merged once, from its first slot
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+        bool placed = false;
+#pragma unroll
+        for (int k = 0; k < l; ++k) {
+            if (!placed && cell[k] == cell[l]) {
+                msk[k] |= 1 << l;
+                placed = true;
             }
-        if (best < 0) break;
-        pos[best]++;
+        }
+        if (!placed) msk[l] |= 1 << l;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        pos[k] = msk[k] ? off[cell[k]] : 0;
+        end[k] = msk[k] ? pos[k] + cnt[cell[k]] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) head[k] = pos[k] < end[k] ? list[pos[k]] : NONE;
+    double acc[CHROMO_MAX_BINDERS + 1];
+#pragma unroll
+    for (int m = 0; m <= CHROMO_MAX_BINDERS; ++m) acc[m] = 0;
+    for (;;) {
+        // the smallest bead index at the buckets' heads; a bead lies in one bucket only
+        int32_t bb = NONE;
+        int bm = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (head[k] < bb) {
+                bb = head[k];
+                bm = msk[k];
+            }
+        }
+        if (bb == NONE) break;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (head[k] == bb) {
+                ++pos[k];
+                head[k] = pos[k] < end[k] ? list[pos[k]] : NONE;
+            }
+        }
         const double wx = wts[bb], wy = wts[N + bb], wz = wts[2 * N + bb];
+        double st[CHROMO_MAX_BINDERS];
+#pragma unroll
+        for (int m = 1; m <= CHROMO_MAX_BINDERS; ++m) st[m - 1] = m < Bp1 ? (double)states[(int64_t)(m - 1) * N + bb] : 0.0;
+#pragma unroll
         for (int l = 0; l < 8; ++l) {
-            if (!(masks[best] >> l & 1)) continue;
-            double d = corner_weight(l, wx, wy, wz) / G.vol_bin;
+            if (!(bm >> l & 1)) continue;
+            const double d = corner_weight(l, wx, wy, wz) / G.vol_bin;
             acc[0] += d;
-            for (int m = 1; m < Bp1; ++m) acc[m] += d * (double)states[(int64_t)(m - 1) * N + bb];
+#pragma unroll
+            for (int m = 1; m <= CHROMO_MAX_BINDERS; ++m)
+                if (m < Bp1) acc[m] += d * st[m - 1];
         }
     }
-    for (int m = 0; m < Bp1; ++m) {
-        double a = acc[m];
-        if (cleanup && fabs(a) < 1E-18) a = 0;
-        density[v * Bp1 + m] = a;
+#pragma unroll
+    for (int m = 0; m <= CHROMO_MAX_BINDERS; ++m) {
+        if (m < Bp1) {
+            double a = acc[m];
+            if (cleanup && fabs(a) < 1E-18) a = 0;
+            density[v * Bp1 + m] = a;
+        }
     }
 }
 

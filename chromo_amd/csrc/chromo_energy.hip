@@ -253,24 +253,36 @@ __global__ void k_e_bead(EModel M, const double *theta, const double *r, const d
     block_reduce_store(acc, nt, part + (int64_t)blockIdx.x * MAXT);
 }
 
-// Final single-block reduction over block partials (fixed order) and assembly.
-__global__ void k_e_final(EModel M, const double *theta, const double *vpart, int nbv, int ntv,
-                          const double *bpart, int nbb, int ntb, double *E_out, double *gtheta)
+// Sums of the block partials of ntv voxel terms and ntb bead terms, by one block of
+// FINAL_THREADS: wavefront w takes terms w, w + 16, ...; lane i adds partials i,
+// i + 64, ... in increasing order, then a butterfly over the 64 lanes.  The order
+// depends on the sizes only, so the sums are the same on every run.
+constexpr int FINAL_THREADS = 1024;
+
+__device__ void final_sums(const double *vpart, int nbv, int ntv, const double *bpart, int nbb, int ntb,
+                           double *sv, double *sb)
 {
-    __shared__ double sv[MAXT], sb[MAXT];
-    int t = threadIdx.x;
-    if (t < ntv) {
+    const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+    for (int q = wv; q < ntv + ntb; q += FINAL_THREADS / 64) {
+        const bool vox = q < ntv;
+        const double *p = vox ? vpart : bpart;
+        const int nbk = vox ? nbv : nbb, t = vox ? q : q - ntv;
         double s = 0;
-        for (int i = 0; i < nbv; ++i) s += vpart[(int64_t)i * MAXT + t];
-        sv[t] = s;
-    }
-    if (t < ntb) {
-        double s = 0;
-        for (int i = 0; i < nbb; ++i) s += bpart[(int64_t)i * MAXT + t];
-        sb[t] = s;
+        for (int i = lane; i < nbk; i += 64) s += p[(int64_t)i * MAXT + t];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == 0) (vox ? sv : sb)[t] = s;
     }
     __syncthreads();
-    if (t != 0) return;
+}
+
+// Final reduction over block partials (fixed order) and assembly.
+__global__ void __launch_bounds__(FINAL_THREADS)
+k_e_final(EModel M, const double *theta, const double *vpart, int nbv, int ntv, const double *bpart, int nbb,
+          int ntb, double *E_out, double *gtheta)
+{
+    __shared__ double sv[MAXT], sb[MAXT];
+    final_sums(vpart, nbv, ntv, bpart, nbb, ntb, sv, sb);
+    if (threadIdx.x != 0) return;
     const int B = M.B;
     const Coef c = coef_of(M, theta);
     const double vb = M.G.vol_bin;
@@ -380,24 +392,14 @@ __global__ void k_ref_bead(EModel M, const double *pos, int64_t sc, int64_t sb, 
     block_reduce_store(acc, nt, part + (int64_t)blockIdx.x * MAXT);
 }
 
-__global__ void k_ref_final(EModel M, const double *pref, const double *intra, const double *vpart,
-                            int nbv, const double *bpart, int nbb, double *out)
+__global__ void __launch_bounds__(FINAL_THREADS)
+k_ref_final(EModel M, const double *pref, const double *intra, const double *vpart, int nbv,
+            const double *bpart, int nbb, double *out)
 {
     __shared__ double sv[MAXT], sb[MAXT];
-    int t = threadIdx.x;
     const int B = M.B;
-    if (t < B + 1) {
-        double s = 0;
-        for (int i = 0; i < nbv; ++i) s += vpart[(int64_t)i * MAXT + t];
-        sv[t] = s;
-    }
-    if (t < B + 1) {
-        double s = 0;
-        for (int i = 0; i < nbb; ++i) s += bpart[(int64_t)i * MAXT + t];
-        sb[t] = s;
-    }
-    __syncthreads();
-    if (t != 0) return;
+    final_sums(vpart, nbv, B + 1, bpart, nbb, B + 1, sv, sb);
+    if (threadIdx.x != 0) return;
     double E = 0;
     for (int i = 0; i < B; ++i) {
         E += pref[i] * sv[i];
@@ -481,7 +483,7 @@ int chromo_energy_adjoint(const chromo_energy_model *model, const double *d_r, c
     const int ntv = B + B * (B - 1) / 2 + 2, ntb = 2 + 2 * B;
     hipLaunchKernelGGL(k_e_voxel, dim3(gv), dim3(EB), 0, st, M, d_theta, rho, dEdrho, vpart, ntv);
     hipLaunchKernelGGL(k_e_bead, dim3(gb), dim3(EB), 0, st, M, d_theta, d_r, dEdrho, d_grad_r, bpart, ntb);
-    hipLaunchKernelGGL(k_e_final, dim3(1), dim3(64), 0, st, M, d_theta, vpart, gv, ntv, bpart, gb, ntb,
+    hipLaunchKernelGGL(k_e_final, dim3(1), dim3(FINAL_THREADS), 0, st, M, d_theta, vpart, gv, ntv, bpart, gb, ntb,
                        d_E, d_grad_theta);
     HIPE(hipGetLastError());
     density_scratch_free(S, st);
@@ -517,7 +519,7 @@ int chromo_energy_reference(const chromo_energy_model *model, const double *d_po
     if (e != hipSuccess) return eset(-2, std::string("density scatter: ") + hipGetErrorString(e));
     hipLaunchKernelGGL(k_ref_voxel, dim3(gv), dim3(EB), 0, st, M, d_pref, chi, rho, vpart, B + 1);
     hipLaunchKernelGGL(k_ref_bead, dim3(gb), dim3(EB), 0, st, M, d_pos, sc, sbd, bpart, B + 1);
-    hipLaunchKernelGGL(k_ref_final, dim3(1), dim3(64), 0, st, M, d_pref, d_intra, vpart, gv, bpart, gb,
+    hipLaunchKernelGGL(k_ref_final, dim3(1), dim3(FINAL_THREADS), 0, st, M, d_pref, d_intra, vpart, gv, bpart, gb,
                        d_out);
     HIPE(hipGetLastError());
     density_scratch_free(S, st);
