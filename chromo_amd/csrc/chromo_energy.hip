@@ -98,8 +98,53 @@ __device__ inline Coef coef_of(const EModel &M, const double *th)
     return c;
 }
 
-// fixed-order block sum of MAXT-wide partial vectors
-__device__ void block_reduce_store(double *vals, int nt, double *out_block)
+// coef_of for a binder count known at compile time: the same arithmetic, with
+// constant indices (registers instead of scratch in the per-voxel and per-bead loops)
+template <int B>
+__device__ inline Coef coef_of(const EModel &M, const double *th)
+{
+    Coef c;
+    const double vb = M.G.vol_bin;
+    c.chi = th[0];
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+        double e = th[eps_index(B, j, j)];
+        c.pref[j] = 0.5 * e * M.v_int[j] * vb;
+        c.intra[j] = e * (1 - M.v_int[j] / vb);
+        c.mu[j] = th[1 + B * (B + 1) / 2 + j];
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            c.X[j * B + k] = M.cross_mask[j * B + k] ? th[eps_index(B, j, k)] * M.v_int[j] * vb : 0.0;
+    }
+    return c;
+}
+
+// Block partials are stored term-major: part[t * gridDim.x + block], so that the
+// final sums read each term's partials contiguously.
+
+// Fixed-order block sum of NT partial values per thread: a butterfly within each
+// wavefront, then the EB / 64 wavefront sums in wavefront order.
+template <int NT>
+__device__ void block_reduce_store(const double (&vals)[NT], double *part)
+{
+    __shared__ double sh[EB / 64][NT];
+    const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        double s = vals[t];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == 0) sh[wv][t] = s;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < NT) {
+        double s = sh[0][threadIdx.x];
+        for (int w = 1; w < EB / 64; ++w) s += sh[w][threadIdx.x];
+        part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// fixed-order block sum of nt partial values per thread
+__device__ void block_reduce_store(double *vals, int nt, double *part)
 {
     __shared__ double sh[EB];
     for (int t = 0; t < nt; ++t) {
@@ -109,27 +154,33 @@ __device__ void block_reduce_store(double *vals, int nt, double *out_block)
             if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
             __syncthreads();
         }
-        if (threadIdx.x == 0) out_block[t] = sh[0];
+        if (threadIdx.x == 0) part[(int64_t)t * gridDim.x + blockIdx.x] = sh[0];
         __syncthreads();
     }
 }
 
 // ---- voxel terms -----------------------------------------------------------
 // slots: [0..B) S_jj, [B..B+B(B-1)/2) S_jk (j<k), then sum g, sum dg/dchi
-__global__ void k_e_voxel(EModel M, const double *theta, const double *rho, double *dEdrho,
-                          double *part, int nt)
+template <int B>
+__global__ void __launch_bounds__(EB) k_e_voxel(EModel M, const double *theta, const double *rho, double *dEdrho,
+                                                double *part)
 {
-    const int B = M.B, Bp1 = B + 1;
-    const Coef c = coef_of(M, theta);
+    constexpr int Bp1 = B + 1, NT = B + B * (B - 1) / 2 + 2;
+    const Coef c = coef_of<B>(M, theta);
     const int64_t nb = M.G.nx * M.G.ny * M.G.nz;
-    double acc[MAXT];
-    for (int t = 0; t < nt; ++t) acc[t] = 0;
+    double acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = 0;
     const double V = M.bead_vol, vb = M.G.vol_bin;
     for (int64_t v = (int64_t)blockIdx.x * EB + threadIdx.x; v < nb; v += (int64_t)gridDim.x * EB) {
-        const double *r = rho + v * Bp1;
+        double r[Bp1];
+#pragma unroll
+        for (int j = 0; j < Bp1; ++j) r[j] = rho[v * Bp1 + j];
         int s = B;
+#pragma unroll
         for (int j = 0; j < B; ++j) {
             acc[j] += r[j + 1] * r[j + 1];
+#pragma unroll
             for (int k = j + 1; k < B; ++k) acc[s++] += r[j + 1] * r[k + 1];
         }
         double phi = r[0] * V;
@@ -139,14 +190,16 @@ __global__ void k_e_voxel(EModel M, const double *theta, const double *rho, doub
         if (dEdrho) {
             double *d = dEdrho + v * Bp1;
             d[0] = V * (huge ? E_HUGE_FIELD : 2 * c.chi * (vb / V) * phi);
+#pragma unroll
             for (int j = 0; j < B; ++j) {
                 double g = 2 * c.pref[j] * r[j + 1];
+#pragma unroll
                 for (int k = 0; k < B; ++k) g += (c.X[j * B + k] + c.X[k * B + j]) * r[k + 1];
                 d[j + 1] = g;
             }
         }
     }
-    block_reduce_store(acc, nt, part + (int64_t)blockIdx.x * MAXT);
+    block_reduce_store<NT>(acc, part);
 }
 
 // ---- bead / bond terms -------------------------------------------------------
@@ -182,14 +235,16 @@ __device__ inline void E_pair_grad_dr(const Bond &bd, const double *r0, const do
 }
 
 // slots: [0] E_elastic, [1] E_bind, [2..2+B) D_j, [2+B..2+2B) sum s_j
-__global__ void k_e_bead(EModel M, const double *theta, const double *r, const double *dEdrho,
-                         double *grad, double *part, int nt)
+template <int B>
+__global__ void __launch_bounds__(EB) k_e_bead(EModel M, const double *theta, const double *r,
+                                               const double *dEdrho, double *grad, double *part)
 {
-    const int B = M.B, Bp1 = B + 1;
+    constexpr int Bp1 = B + 1, NT = 2 + 2 * B;
     const int64_t N = M.N;
-    const Coef c = coef_of(M, theta);
-    double acc[MAXT];
-    for (int t = 0; t < nt; ++t) acc[t] = 0;
+    const Coef c = coef_of<B>(M, theta);
+    double acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = 0;
     const double f = M.mu_adjust;
     for (int64_t b = (int64_t)blockIdx.x * EB + threadIdx.x; b < N; b += (int64_t)gridDim.x * EB) {
         const double *rb = r + 3 * b;
@@ -198,6 +253,7 @@ __global__ void k_e_bead(EModel M, const double *theta, const double *r, const d
             Bond bd = load_bond(M.bonds, N, b);
             acc[0] += E_pair_abs(bd, rb, r + 3 * (b + 1), tb, M.t3 + 3 * (b + 1));
         }
+#pragma unroll
         for (int j = 0; j < B; ++j) {
             int s = M.states[(int64_t)j * N + b];
             int m = M.marks[(int64_t)j * N + b];
@@ -222,6 +278,7 @@ __global__ void k_e_bead(EModel M, const double *theta, const double *r, const d
                     int64_t v = x + y * M.G.nx + z * M.G.nx * M.G.ny;
                     const double *d = dEdrho + v * Bp1;
                     double h = d[0];
+#pragma unroll
                     for (int j = 0; j < B; ++j) h += (double)M.states[(int64_t)j * N + b] * d[j + 1];
                     h /= M.G.vol_bin;
                     double a[3];
@@ -250,13 +307,23 @@ __global__ void k_e_bead(EModel M, const double *theta, const double *r, const d
             for (int j = 0; j < 3; ++j) grad[3 * b + j] = g[j];
         }
     }
-    block_reduce_store(acc, nt, part + (int64_t)blockIdx.x * MAXT);
+    block_reduce_store<NT>(acc, part);
+}
+
+// The voxel and bead kernels of one energy evaluation, for B binders.
+template <int B>
+void launch_energy_terms(const EModel &M, const double *theta, const double *rho, double *dEdrho, double *vpart,
+                         int gv, const double *r, double *grad, double *bpart, int gb, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_e_voxel<B>, dim3(gv), dim3(EB), 0, st, M, theta, rho, dEdrho, vpart);
+    hipLaunchKernelGGL(k_e_bead<B>, dim3(gb), dim3(EB), 0, st, M, theta, r, dEdrho, grad, bpart);
 }
 
 // Sums of the block partials of ntv voxel terms and ntb bead terms, by one block of
 // FINAL_THREADS: wavefront w takes terms w, w + 16, ...; lane i adds partials i,
-// i + 64, ... in increasing order, then a butterfly over the 64 lanes.  The order
-// depends on the sizes only, so the sums are the same on every run.
+// i + 64, ... in increasing order (loaded 16 at a time, so that one round trip
+// covers 16 of them), then a butterfly over the 64 lanes.  The order depends on
+// the sizes only, so the sums are the same on every run.
 constexpr int FINAL_THREADS = 1024;
 
 __device__ void final_sums(const double *vpart, int nbv, int ntv, const double *bpart, int nbb, int ntb,
@@ -268,14 +335,26 @@ __device__ void final_sums(const double *vpart, int nbv, int ntv, const double *
         const double *p = vox ? vpart : bpart;
         const int nbk = vox ? nbv : nbb, t = vox ? q : q - ntv;
         double s = 0;
-        for (int i = lane; i < nbk; i += 64) s += p[(int64_t)i * MAXT + t];
+        for (int i0 = lane; i0 < nbk; i0 += 64 * 16) {
+            double x[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int i = i0 + 64 * j;
+                x[j] = i < nbk ? p[(int64_t)t * nbk + i] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (i0 + 64 * j < nbk) s += x[j];
+        }
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
         if (lane == 0) (vox ? sv : sb)[t] = s;
     }
     __syncthreads();
 }
 
-// Final reduction over block partials (fixed order) and assembly.
+// Final reduction over block partials (fixed order) and assembly, for B binders
+// (compile-time, so the coefficients and sums stay in registers).
+template <int B>
 __global__ void __launch_bounds__(FINAL_THREADS)
 k_e_final(EModel M, const double *theta, const double *vpart, int nbv, int ntv, const double *bpart, int nbb,
           int ntb, double *E_out, double *gtheta)
@@ -283,13 +362,14 @@ k_e_final(EModel M, const double *theta, const double *vpart, int nbv, int ntv, 
     __shared__ double sv[MAXT], sb[MAXT];
     final_sums(vpart, nbv, ntv, bpart, nbb, ntb, sv, sb);
     if (threadIdx.x != 0) return;
-    const int B = M.B;
-    const Coef c = coef_of(M, theta);
+    const Coef c = coef_of<B>(M, theta);
     const double vb = M.G.vol_bin;
     double S[CHROMO_MAX_BINDERS * CHROMO_MAX_BINDERS];
     int s = B;
+#pragma unroll
     for (int j = 0; j < B; ++j) {
         S[j * B + j] = sv[j];
+#pragma unroll
         for (int k = j + 1; k < B; ++k) {
             S[j * B + k] = sv[s];
             S[k * B + j] = sv[s];
@@ -298,9 +378,11 @@ k_e_final(EModel M, const double *theta, const double *vpart, int nbv, int ntv, 
     }
     const double Eg = sv[s], dchi = sv[s + 1];
     double Ef = 0;
+#pragma unroll
     for (int j = 0; j < B; ++j) {
         Ef += c.pref[j] * S[j * B + j];
         Ef += c.intra[j] * sb[2 + j];
+#pragma unroll
         for (int k = 0; k < B; ++k) Ef += c.X[j * B + k] * S[j * B + k];
     }
     Ef += Eg;
@@ -311,7 +393,9 @@ k_e_final(EModel M, const double *theta, const double *vpart, int nbv, int ntv, 
     E_out[3] = Ebd;
     if (gtheta) {
         gtheta[0] = dchi;
+#pragma unroll
         for (int j = 0; j < B; ++j)
+#pragma unroll
             for (int k = j; k < B; ++k) {
                 double g;
                 if (j == k) {
@@ -323,8 +407,18 @@ k_e_final(EModel M, const double *theta, const double *vpart, int nbv, int ntv, 
                 }
                 gtheta[eps_index(B, j, k)] = g;
             }
+#pragma unroll
         for (int j = 0; j < B; ++j) gtheta[1 + B * (B + 1) / 2 + j] = -sb[2 + B + j];
     }
+}
+
+// The final reduction and assembly of one energy evaluation, for B binders.
+template <int B>
+void launch_energy_final(const EModel &M, const double *theta, const double *vpart, int gv, int ntv,
+                         const double *bpart, int gb, int ntb, double *E, double *gtheta, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_e_final<B>, dim3(1), dim3(FINAL_THREADS), 0, st, M, theta, vpart, gv, ntv, bpart, gb, ntb,
+                       E, gtheta);
 }
 
 // ---- the reference's compute_E (fields.pyx:1939-2286, polymers.pyx:1348-1381) ----
@@ -357,7 +451,7 @@ __global__ void k_ref_voxel(EModel M, const double *pref, double chi, const doub
         if (py_round2(vf) > M.vf_limit) acc[B] += E_HUGE_FIELD * vf;
         else acc[B] += chi * (vb / V) * vf * (1 - vf);
     }
-    block_reduce_store(acc, nt, part + (int64_t)blockIdx.x * MAXT);
+    block_reduce_store(acc, nt, part);
 }
 
 // slots: [0] SSWLC.compute_E bonds, [1..1+B) doubly bound counts
@@ -389,7 +483,7 @@ __global__ void k_ref_bead(EModel M, const double *pos, int64_t sc, int64_t sb, 
         }
         for (int j = 0; j < B; ++j) acc[1 + j] += (M.states[(int64_t)j * N + b] == 2);
     }
-    block_reduce_store(acc, nt, part + (int64_t)blockIdx.x * MAXT);
+    block_reduce_store(acc, nt, part);
 }
 
 __global__ void __launch_bounds__(FINAL_THREADS)
@@ -413,7 +507,7 @@ k_ref_final(EModel M, const double *pref, const double *intra, const double *vpa
 int grid_blocks(int64_t n)
 {
     int64_t g = (n + EB - 1) / EB;
-    if (g > 1024) g = 1024;
+    if (g > 8192) g = 8192;
     if (g < 1) g = 1;
     return (int)g;
 }
@@ -481,10 +575,18 @@ int chromo_energy_adjoint(const chromo_energy_model *model, const double *d_r, c
     hipError_t e = density_scatter(M.G, M.N, Bp1, P, M.states, rho, 0, S, st);
     if (e != hipSuccess) return eset(-2, std::string("density scatter: ") + hipGetErrorString(e));
     const int ntv = B + B * (B - 1) / 2 + 2, ntb = 2 + 2 * B;
-    hipLaunchKernelGGL(k_e_voxel, dim3(gv), dim3(EB), 0, st, M, d_theta, rho, dEdrho, vpart, ntv);
-    hipLaunchKernelGGL(k_e_bead, dim3(gb), dim3(EB), 0, st, M, d_theta, d_r, dEdrho, d_grad_r, bpart, ntb);
-    hipLaunchKernelGGL(k_e_final, dim3(1), dim3(FINAL_THREADS), 0, st, M, d_theta, vpart, gv, ntv, bpart, gb, ntb,
-                       d_E, d_grad_theta);
+    switch (B) {
+    case 1: launch_energy_terms<1>(M, d_theta, rho, dEdrho, vpart, gv, d_r, d_grad_r, bpart, gb, st); break;
+    case 2: launch_energy_terms<2>(M, d_theta, rho, dEdrho, vpart, gv, d_r, d_grad_r, bpart, gb, st); break;
+    case 3: launch_energy_terms<3>(M, d_theta, rho, dEdrho, vpart, gv, d_r, d_grad_r, bpart, gb, st); break;
+    default: launch_energy_terms<4>(M, d_theta, rho, dEdrho, vpart, gv, d_r, d_grad_r, bpart, gb, st); break;
+    }
+    switch (B) {
+    case 1: launch_energy_final<1>(M, d_theta, vpart, gv, ntv, bpart, gb, ntb, d_E, d_grad_theta, st); break;
+    case 2: launch_energy_final<2>(M, d_theta, vpart, gv, ntv, bpart, gb, ntb, d_E, d_grad_theta, st); break;
+    case 3: launch_energy_final<3>(M, d_theta, vpart, gv, ntv, bpart, gb, ntb, d_E, d_grad_theta, st); break;
+    default: launch_energy_final<4>(M, d_theta, vpart, gv, ntv, bpart, gb, ntb, d_E, d_grad_theta, st); break;
+    }
     HIPE(hipGetLastError());
     density_scratch_free(S, st);
     HIPE(hipFreeAsync(rho, st));
