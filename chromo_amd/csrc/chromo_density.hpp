@@ -119,24 +119,13 @@ __global__ void k_dens_sort(int64_t nbins, const int32_t *off, const int32_t *cn
     }
 }
 
-// density[v * Bp1 + m]: m = 0 beads, m >= 1 binder m-1 (weight x state).  Voxel v
-// is corner l of the beads whose lower-corner cell is v - (l & 1, l >> 1 & 1,
-// l >> 2 & 1), wrapped.  Those <= 8 sorted buckets are merged in ascending bead
-// index, and the corners of one bead that land on v (several only on grids one or
-// two voxels wide, where buckets coincide) are folded in corner order: the
-// reference's bead loop.  Every per-bucket value is indexed at compile time, so
-// the merge state stays in registers.
-__global__ void k_dens_gather(GridGeom G, int64_t N, int Bp1, const int8_t *states,
-                              const int32_t *off, const int32_t *cnt, const int32_t *list,
-                              const double *wts, double *density, int cleanup)
+// The buckets that reach voxel v: cell[l] holds the beads of which v is corner l;
+// a bucket reached by several corners (grids one or two voxels wide) is kept once,
+// in its first slot, whose mask collects every such corner.
+__device__ __forceinline__ void voxel_cells(const GridGeom &G, int64_t v, int32_t (&cell)[8], int (&msk)[8])
 {
-    constexpr int32_t NONE = 0x7fffffff;
-    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nx = G.nx, ny = G.ny, nz = G.nz;
-    if (v >= nx * ny * nz) return;
     const int64_t x = v % nx, y = (v / nx) % ny, z = v / (nx * ny);
-    int32_t cell[8], pos[8], end[8], head[8];
-    int msk[8];
 #pragma unroll
     for (int l = 0; l < 8; ++l) {
         int64_t cx = x - (l & 1), cy = y - ((l >> 1) & 1), cz = z - ((l >> 2) & 1);
@@ -146,7 +135,6 @@ __global__ void k_dens_gather(GridGeom G, int64_t N, int Bp1, const int8_t *stat
         cell[l] = (int32_t)(cx + cy * nx + cz * nx * ny);
         msk[l] = 0;
     }
-    // a bucket reached by several corners is merged once, from its first slot
 #pragma unroll
     for (int l = 0; l < 8; ++l) {
         bool placed = false;
@@ -159,6 +147,67 @@ __global__ void k_dens_gather(GridGeom G, int64_t N, int Bp1, const int8_t *stat
         }
         if (!placed) msk[l] |= 1 << l;
     }
+}
+
+// On large grids (at least GATHER_BIG_BLOCKS blocks of 1024 voxels) one gather
+// block takes 1024 consecutive voxels and hands them to its threads in order of
+// merge length (the beads of their buckets, keys clamped to GATHER_KEYS - 1), so
+// that the lanes of a wavefront have merges of similar length.  Smaller grids run
+// in one round of 256-thread blocks, voxel t of the block on thread t: there the
+// ranking's extra phase costs more than the balance saves.
+constexpr int GATHER_KEYS = 64, GATHER_BIG_BLOCKS = 512;
+
+// density[v * Bp1 + m]: m = 0 beads, m >= 1 binder m-1 (weight x state).  Voxel v
+// is corner l of the beads whose lower-corner cell is v - (l & 1, l >> 1 & 1,
+// l >> 2 & 1), wrapped.  Those <= 8 sorted buckets are merged in ascending bead
+// index, and the corners of one bead that land on v (several only on grids one or
+// two voxels wide, where buckets coincide) are folded in corner order: the
+// reference's bead loop.  Every per-bucket value is indexed at compile time, so
+// the merge state stays in registers.  Which thread merges which voxel of the
+// block does not change any voxel's sum.
+template <int T>
+__global__ void __launch_bounds__(T)
+k_dens_gather(GridGeom G, int64_t N, int Bp1, const int8_t *states, const int32_t *off, const int32_t *cnt,
+              const int32_t *list, const double *wts, double *density, int cleanup)
+{
+    constexpr int32_t NONE = 0x7fffffff;
+    __shared__ int32_t hist[GATHER_KEYS], hoff[GATHER_KEYS], perm[T];
+    const int t = threadIdx.x;
+    const int64_t nbv = G.nx * G.ny * G.nz, base = (int64_t)blockIdx.x * T;
+    const int nreal = (int)(nbv - base < T ? nbv - base : T);
+    int32_t cell[8], pos[8], end[8], head[8];
+    int msk[8];
+    int64_t v = base + t;
+    if constexpr (T == 1024) {
+        if (t < GATHER_KEYS) hist[t] = 0;
+        __syncthreads();
+        int key = 0;
+        if (t < nreal) {
+            voxel_cells(G, base + t, cell, msk);
+            int64_t n = 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) n += msk[k] ? cnt[cell[k]] : 0;
+            key = (int)(n < GATHER_KEYS - 1 ? n : GATHER_KEYS - 1);
+            atomicAdd(&hist[key], 1);
+        }
+        __syncthreads();
+        if (t < GATHER_KEYS) {
+            static_assert(GATHER_KEYS == 64, "one wavefront scans the key counts");
+            const int c = hist[t];
+            int incl = c;
+            for (int o = 1; o < GATHER_KEYS; o <<= 1) {
+                const int y = __shfl_up(incl, o);
+                if (t >= o) incl += y;
+            }
+            hoff[t] = incl - c;
+        }
+        __syncthreads();
+        if (t < nreal) perm[atomicAdd(&hoff[key], 1)] = t;
+        __syncthreads();
+        if (t < nreal) v = base + perm[t];
+    }
+    if (t >= nreal) return;
+    voxel_cells(G, v, cell, msk);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         pos[k] = msk[k] ? off[cell[k]] : 0;
@@ -269,8 +318,12 @@ inline hipError_t density_scatter(const GridGeom &G, int64_t N, int Bp1, PosView
     hipLaunchKernelGGL(k_scan_add, dim3(gv), dim3(256), 0, st, S.off, S.boff, nb);
     hipLaunchKernelGGL(k_dens_place, dim3(gb), dim3(256), 0, st, N, S.cell, S.off, S.fill, S.list);
     hipLaunchKernelGGL(k_dens_sort, dim3(gv), dim3(256), 0, st, nb, S.off, S.cnt, S.list);
-    hipLaunchKernelGGL(k_dens_gather, dim3(gv), dim3(256), 0, st, G, N, Bp1, states, S.off, S.cnt,
-                       S.list, S.wts, density, cleanup);
+    if (nb >= (int64_t)1024 * GATHER_BIG_BLOCKS)
+        hipLaunchKernelGGL(k_dens_gather<1024>, dim3((unsigned)((nb + 1023) / 1024)), dim3(1024), 0, st, G, N, Bp1,
+                           states, S.off, S.cnt, S.list, S.wts, density, cleanup);
+    else
+        hipLaunchKernelGGL(k_dens_gather<256>, dim3(gv), dim3(256), 0, st, G, N, Bp1, states, S.off, S.cnt, S.list,
+                           S.wts, density, cleanup);
     return hipGetLastError();
 }
 
