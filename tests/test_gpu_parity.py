@@ -53,8 +53,10 @@ def _dE_close(a, b):
 
 # ---------------------------------------------------------------- density ---
 
+# 96^3 voxels reach the gather that ranks voxels inside 1024-voxel blocks (grids of
+# at least 512 such blocks); the others use 256-voxel blocks
 @pytest.mark.parametrize("N,nvox,seed", [(300, 7, 0), (2000, 16, 1), (5000, 12, 2), (64, 1, 3),
-                                         (64, 2, 4), (1000, 5, 5)])
+                                         (64, 2, 4), (1000, 5, 5), (200000, 96, 6)])
 def test_density_init_bitwise(oracle, N, nvox, seed):
     say("density_init", N, nvox, seed)
     s = W.make_system(N, 2, nvox, seed=seed)
