@@ -235,9 +235,12 @@ __device__ inline void E_pair_grad_dr(const Bond &bd, const double *r0, const do
 }
 
 // slots: [0] E_elastic, [1] E_bind, [2..2+B) D_j, [2+B..2+2B) sum s_j
+// cell and wts: each bead's lower-corner voxel and weights from the density
+// scatter of the same positions (grid_lower's values, not recomputed).
 template <int B>
 __global__ void __launch_bounds__(EB) k_e_bead(EModel M, const double *theta, const double *r,
-                                               const double *dEdrho, double *grad, double *part)
+                                               const double *dEdrho, const int32_t *cell, const double *wts,
+                                               double *grad, double *part)
 {
     constexpr int Bp1 = B + 1, NT = 2 + 2 * B;
     const int64_t N = M.N;
@@ -267,9 +270,9 @@ __global__ void __launch_bounds__(EB) k_e_bead(EModel M, const double *theta, co
         if (grad) {
             double g[3] = {0, 0, 0};
             if (dEdrho) {
-                int64_t ix[3];
-                double w[3];
-                grid_lower(M.G, rb[0], rb[1], rb[2], ix, w);
+                const int64_t c = cell[b];
+                const int64_t ix[3] = {c % M.G.nx, (c / M.G.nx) % M.G.ny, c / (M.G.nx * M.G.ny)};
+                const double w[3] = {wts[b], wts[N + b], wts[2 * N + b]};
                 for (int l = 0; l < 8; ++l) {
                     int64_t x = ix[0] + (l & 1), y = ix[1] + ((l >> 1) & 1), z = ix[2] + ((l >> 2) & 1);
                     if (x >= M.G.nx) x -= M.G.nx;
@@ -313,10 +316,11 @@ __global__ void __launch_bounds__(EB) k_e_bead(EModel M, const double *theta, co
 // The voxel and bead kernels of one energy evaluation, for B binders.
 template <int B>
 void launch_energy_terms(const EModel &M, const double *theta, const double *rho, double *dEdrho, double *vpart,
-                         int gv, const double *r, double *grad, double *bpart, int gb, hipStream_t st)
+                         int gv, const double *r, const DensityScratch &S, double *grad, double *bpart, int gb,
+                         hipStream_t st)
 {
     hipLaunchKernelGGL(k_e_voxel<B>, dim3(gv), dim3(EB), 0, st, M, theta, rho, dEdrho, vpart);
-    hipLaunchKernelGGL(k_e_bead<B>, dim3(gb), dim3(EB), 0, st, M, theta, r, dEdrho, grad, bpart);
+    hipLaunchKernelGGL(k_e_bead<B>, dim3(gb), dim3(EB), 0, st, M, theta, r, dEdrho, S.cell, S.wts, grad, bpart);
 }
 
 // Sums of the block partials of ntv voxel terms and ntb bead terms, by one block of
@@ -576,10 +580,10 @@ int chromo_energy_adjoint(const chromo_energy_model *model, const double *d_r, c
     if (e != hipSuccess) return eset(-2, std::string("density scatter: ") + hipGetErrorString(e));
     const int ntv = B + B * (B - 1) / 2 + 2, ntb = 2 + 2 * B;
     switch (B) {
-    case 1: launch_energy_terms<1>(M, d_theta, rho, dEdrho, vpart, gv, d_r, d_grad_r, bpart, gb, st); break;
-    case 2: launch_energy_terms<2>(M, d_theta, rho, dEdrho, vpart, gv, d_r, d_grad_r, bpart, gb, st); break;
-    case 3: launch_energy_terms<3>(M, d_theta, rho, dEdrho, vpart, gv, d_r, d_grad_r, bpart, gb, st); break;
-    default: launch_energy_terms<4>(M, d_theta, rho, dEdrho, vpart, gv, d_r, d_grad_r, bpart, gb, st); break;
+    case 1: launch_energy_terms<1>(M, d_theta, rho, dEdrho, vpart, gv, d_r, S, d_grad_r, bpart, gb, st); break;
+    case 2: launch_energy_terms<2>(M, d_theta, rho, dEdrho, vpart, gv, d_r, S, d_grad_r, bpart, gb, st); break;
+    case 3: launch_energy_terms<3>(M, d_theta, rho, dEdrho, vpart, gv, d_r, S, d_grad_r, bpart, gb, st); break;
+    default: launch_energy_terms<4>(M, d_theta, rho, dEdrho, vpart, gv, d_r, S, d_grad_r, bpart, gb, st); break;
     }
     switch (B) {
     case 1: launch_energy_final<1>(M, d_theta, vpart, gv, ntv, bpart, gb, ntb, d_E, d_grad_theta, st); break;
