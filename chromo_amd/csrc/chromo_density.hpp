@@ -47,8 +47,17 @@ __device__ __forceinline__ void grid_lower(const GridGeom &G, double x0, double 
     }
 }
 
-__global__ void k_dens_cells(GridGeom G, int64_t N, PosView P, int32_t *cell, double *wts,
-                             int32_t *cnt)
+// What the gather reads of one bead, in one 32-byte record (one cache line per
+// merge step instead of one per weight and per state row).
+struct BeadRec {
+    double w[3];
+    int8_t st[CHROMO_MAX_BINDERS];
+    int32_t pad;
+};
+static_assert(sizeof(BeadRec) == 32, "one bead record is 32 bytes");
+
+__global__ void k_dens_cells(GridGeom G, int64_t N, PosView P, int B, const int8_t *states, int32_t *cell,
+                             double *wts, BeadRec *brec, int32_t *cnt)
 {
     int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= N) return;
@@ -60,6 +69,14 @@ __global__ void k_dens_cells(GridGeom G, int64_t N, PosView P, int32_t *cell, do
     wts[b] = w[0];
     wts[N + b] = w[1];
     wts[2 * N + b] = w[2];
+    BeadRec q;
+    q.w[0] = w[0];
+    q.w[1] = w[1];
+    q.w[2] = w[2];
+#pragma unroll
+    for (int m = 0; m < CHROMO_MAX_BINDERS; ++m) q.st[m] = m < B ? states[(int64_t)m * N + b] : (int8_t)0;
+    q.pad = 0;
+    brec[b] = q;
     atomicAdd(&cnt[c], 1);
 }
 
@@ -167,8 +184,8 @@ constexpr int GATHER_KEYS = 64, GATHER_BIG_BLOCKS = 512;
 // block does not change any voxel's sum.
 template <int T>
 __global__ void __launch_bounds__(T)
-k_dens_gather(GridGeom G, int64_t N, int Bp1, const int8_t *states, const int32_t *off, const int32_t *cnt,
-              const int32_t *list, const double *wts, double *density, int cleanup)
+k_dens_gather(GridGeom G, int Bp1, const int32_t *off, const int32_t *cnt, const int32_t *list,
+              const BeadRec *brec, double *density, int cleanup)
 {
     constexpr int32_t NONE = 0x7fffffff;
     __shared__ int32_t hist[GATHER_KEYS], hoff[GATHER_KEYS], perm[T];
@@ -237,10 +254,11 @@ k_dens_gather(GridGeom G, int64_t N, int Bp1, const int8_t *states, const int32_
                 head[k] = pos[k] < end[k] ? list[pos[k]] : NONE;
             }
         }
-        const double wx = wts[bb], wy = wts[N + bb], wz = wts[2 * N + bb];
+        const BeadRec q = brec[bb];
+        const double wx = q.w[0], wy = q.w[1], wz = q.w[2];
         double st[CHROMO_MAX_BINDERS];
 #pragma unroll
-        for (int m = 1; m <= CHROMO_MAX_BINDERS; ++m) st[m - 1] = m < Bp1 ? (double)states[(int64_t)(m - 1) * N + bb] : 0.0;
+        for (int m = 1; m <= CHROMO_MAX_BINDERS; ++m) st[m - 1] = m < Bp1 ? (double)q.st[m - 1] : 0.0;
 #pragma unroll
         for (int l = 0; l < 8; ++l) {
             if (!(bm >> l & 1)) continue;
@@ -266,6 +284,7 @@ struct DensityScratch {
     int32_t *cell = nullptr, *list = nullptr, *cnt = nullptr, *off = nullptr, *fill = nullptr;
     int32_t *bsum = nullptr, *boff = nullptr, *bsum2 = nullptr;
     double *wts = nullptr;
+    BeadRec *brec = nullptr;
     int64_t maxN = 0, maxBins = 0;
 };
 
@@ -280,6 +299,7 @@ inline hipError_t density_scratch_alloc(DensityScratch &S, int64_t maxN, int64_t
     CHROMO_ALLOC(S.cell, sizeof(int32_t) * maxN);
     CHROMO_ALLOC(S.list, sizeof(int32_t) * maxN);
     CHROMO_ALLOC(S.wts, sizeof(double) * 3 * maxN);
+    CHROMO_ALLOC(S.brec, sizeof(BeadRec) * maxN);
     CHROMO_ALLOC(S.cnt, sizeof(int32_t) * maxBins);
     CHROMO_ALLOC(S.off, sizeof(int32_t) * maxBins);
     CHROMO_ALLOC(S.fill, sizeof(int32_t) * maxBins);
@@ -296,6 +316,7 @@ inline void density_scratch_free(DensityScratch &S, hipStream_t st)
     for (auto p : ip)
         if (p) (void)hipFreeAsync(p, st);
     if (S.wts) (void)hipFreeAsync(S.wts, st);
+    if (S.brec) (void)hipFreeAsync(S.brec, st);
     S = DensityScratch();
 }
 
@@ -311,7 +332,8 @@ inline hipError_t density_scatter(const GridGeom &G, int64_t N, int Bp1, PosView
     if ((e = hipMemsetAsync(S.fill, 0, sizeof(int32_t) * nb, st)) != hipSuccess) return e;
     unsigned gb = (unsigned)((N + 255) / 256);
     unsigned gv = (unsigned)((nb + 255) / 256);
-    hipLaunchKernelGGL(k_dens_cells, dim3(gb), dim3(256), 0, st, G, N, P, S.cell, S.wts, S.cnt);
+    hipLaunchKernelGGL(k_dens_cells, dim3(gb), dim3(256), 0, st, G, N, P, Bp1 - 1, states, S.cell, S.wts, S.brec,
+                       S.cnt);
     int64_t nbl = (nb + 2047) / 2048;
     hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nbl), dim3(1024), 0, st, S.cnt, S.off, S.bsum, nb);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, S.bsum, S.boff, S.bsum2, nbl);
@@ -319,11 +341,11 @@ inline hipError_t density_scatter(const GridGeom &G, int64_t N, int Bp1, PosView
     hipLaunchKernelGGL(k_dens_place, dim3(gb), dim3(256), 0, st, N, S.cell, S.off, S.fill, S.list);
     hipLaunchKernelGGL(k_dens_sort, dim3(gv), dim3(256), 0, st, nb, S.off, S.cnt, S.list);
     if (nb >= (int64_t)1024 * GATHER_BIG_BLOCKS)
-        hipLaunchKernelGGL(k_dens_gather<1024>, dim3((unsigned)((nb + 1023) / 1024)), dim3(1024), 0, st, G, N, Bp1,
-                           states, S.off, S.cnt, S.list, S.wts, density, cleanup);
+        hipLaunchKernelGGL(k_dens_gather<1024>, dim3((unsigned)((nb + 1023) / 1024)), dim3(1024), 0, st, G, Bp1,
+                           S.off, S.cnt, S.list, S.brec, density, cleanup);
     else
-        hipLaunchKernelGGL(k_dens_gather<256>, dim3(gv), dim3(256), 0, st, G, N, Bp1, states, S.off, S.cnt, S.list,
-                           S.wts, density, cleanup);
+        hipLaunchKernelGGL(k_dens_gather<256>, dim3(gv), dim3(256), 0, st, G, Bp1, S.off, S.cnt, S.list, S.brec,
+                           density, cleanup);
     return hipGetLastError();
 }
 
