@@ -57,7 +57,7 @@ struct BeadRec {
 static_assert(sizeof(BeadRec) == 32, "one bead record is 32 bytes");
 
 __global__ void k_dens_cells(GridGeom G, int64_t N, PosView P, int B, const int8_t *states, int32_t *cell,
-                             double *wts, BeadRec *brec, int32_t *cnt)
+                             BeadRec *brec, int32_t *cnt)
 {
     int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= N) return;
@@ -66,9 +66,6 @@ __global__ void k_dens_cells(GridGeom G, int64_t N, PosView P, int B, const int8
     grid_lower(G, P.p[b * P.sb], P.p[P.sc + b * P.sb], P.p[2 * P.sc + b * P.sb], ix, w);
     int32_t c = (int32_t)(ix[0] + ix[1] * G.nx + ix[2] * G.nx * G.ny);
     cell[b] = c;
-    wts[b] = w[0];
-    wts[N + b] = w[1];
-    wts[2 * N + b] = w[2];
     BeadRec q;
     q.w[0] = w[0];
     q.w[1] = w[1];
@@ -283,7 +280,6 @@ k_dens_gather(GridGeom G, int Bp1, const int32_t *off, const int32_t *cnt, const
 struct DensityScratch {
     int32_t *cell = nullptr, *list = nullptr, *cnt = nullptr, *off = nullptr, *fill = nullptr;
     int32_t *bsum = nullptr, *boff = nullptr, *bsum2 = nullptr;
-    double *wts = nullptr;
     BeadRec *brec = nullptr;
     int64_t maxN = 0, maxBins = 0;
 };
@@ -298,7 +294,6 @@ inline hipError_t density_scratch_alloc(DensityScratch &S, int64_t maxN, int64_t
     if ((e = hipMallocAsync((void **)&ptr, bytes, st)) != hipSuccess) return e
     CHROMO_ALLOC(S.cell, sizeof(int32_t) * maxN);
     CHROMO_ALLOC(S.list, sizeof(int32_t) * maxN);
-    CHROMO_ALLOC(S.wts, sizeof(double) * 3 * maxN);
     CHROMO_ALLOC(S.brec, sizeof(BeadRec) * maxN);
     CHROMO_ALLOC(S.cnt, sizeof(int32_t) * maxBins);
     CHROMO_ALLOC(S.off, sizeof(int32_t) * maxBins);
@@ -315,7 +310,6 @@ inline void density_scratch_free(DensityScratch &S, hipStream_t st)
     int32_t *ip[] = {S.cell, S.list, S.cnt, S.off, S.fill, S.bsum, S.boff, S.bsum2};
     for (auto p : ip)
         if (p) (void)hipFreeAsync(p, st);
-    if (S.wts) (void)hipFreeAsync(S.wts, st);
     if (S.brec) (void)hipFreeAsync(S.brec, st);
     S = DensityScratch();
 }
@@ -332,7 +326,7 @@ inline hipError_t density_scatter(const GridGeom &G, int64_t N, int Bp1, PosView
     if ((e = hipMemsetAsync(S.fill, 0, sizeof(int32_t) * nb, st)) != hipSuccess) return e;
     unsigned gb = (unsigned)((N + 255) / 256);
     unsigned gv = (unsigned)((nb + 255) / 256);
-    hipLaunchKernelGGL(k_dens_cells, dim3(gb), dim3(256), 0, st, G, N, P, Bp1 - 1, states, S.cell, S.wts, S.brec,
+    hipLaunchKernelGGL(k_dens_cells, dim3(gb), dim3(256), 0, st, G, N, P, Bp1 - 1, states, S.cell, S.brec,
                        S.cnt);
     int64_t nbl = (nb + 2047) / 2048;
     hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nbl), dim3(1024), 0, st, S.cnt, S.off, S.bsum, nb);

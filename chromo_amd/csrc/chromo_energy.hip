@@ -235,11 +235,11 @@ __device__ inline void E_pair_grad_dr(const Bond &bd, const double *r0, const do
 }
 
 // slots: [0] E_elastic, [1] E_bind, [2..2+B) D_j, [2+B..2+2B) sum s_j
-// cell and wts: each bead's lower-corner voxel and weights from the density
+// cell and brec: each bead's lower-corner voxel and weights from the density
 // scatter of the same positions (grid_lower's values, not recomputed).
 template <int B>
 __global__ void __launch_bounds__(EB) k_e_bead(EModel M, const double *theta, const double *r,
-                                               const double *dEdrho, const int32_t *cell, const double *wts,
+                                               const double *dEdrho, const int32_t *cell, const BeadRec *brec,
                                                double *grad, double *part)
 {
     constexpr int Bp1 = B + 1, NT = 2 + 2 * B;
@@ -272,7 +272,7 @@ __global__ void __launch_bounds__(EB) k_e_bead(EModel M, const double *theta, co
             if (dEdrho) {
                 const int64_t c = cell[b];
                 const int64_t ix[3] = {c % M.G.nx, (c / M.G.nx) % M.G.ny, c / (M.G.nx * M.G.ny)};
-                const double w[3] = {wts[b], wts[N + b], wts[2 * N + b]};
+                const double w[3] = {brec[b].w[0], brec[b].w[1], brec[b].w[2]};
                 for (int l = 0; l < 8; ++l) {
                     int64_t x = ix[0] + (l & 1), y = ix[1] + ((l >> 1) & 1), z = ix[2] + ((l >> 2) & 1);
                     if (x >= M.G.nx) x -= M.G.nx;
@@ -320,7 +320,7 @@ void launch_energy_terms(const EModel &M, const double *theta, const double *rho
                          hipStream_t st)
 {
     hipLaunchKernelGGL(k_e_voxel<B>, dim3(gv), dim3(EB), 0, st, M, theta, rho, dEdrho, vpart);
-    hipLaunchKernelGGL(k_e_bead<B>, dim3(gb), dim3(EB), 0, st, M, theta, r, dEdrho, S.cell, S.wts, grad, bpart);
+    hipLaunchKernelGGL(k_e_bead<B>, dim3(gb), dim3(EB), 0, st, M, theta, r, dEdrho, S.cell, S.brec, grad, bpart);
 }
 
 // Sums of the block partials of ntv voxel terms and ntb bead terms, by one block of
