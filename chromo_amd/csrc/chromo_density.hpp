@@ -179,7 +179,7 @@ constexpr int GATHER_KEYS = 64, GATHER_BIG_BLOCKS = 512;
 // reference's bead loop.  Every per-bucket value is indexed at compile time, so
 // the merge state stays in registers.  Which thread merges which voxel of the
 // block does not change any voxel's sum.
-template <int T>
+template <int T, bool RANK>
 __global__ void __launch_bounds__(T)
 k_dens_gather(GridGeom G, int Bp1, const int32_t *off, const int32_t *cnt, const int32_t *list,
               const BeadRec *brec, double *density, int cleanup)
@@ -192,7 +192,7 @@ k_dens_gather(GridGeom G, int Bp1, const int32_t *off, const int32_t *cnt, const
     int32_t cell[8], pos[8], end[8], head[8];
     int msk[8];
     int64_t v = base + t;
-    if constexpr (T == 1024) {
+    if constexpr (RANK) {
         if (t < GATHER_KEYS) hist[t] = 0;
         __syncthreads();
         int key = 0;
@@ -335,10 +335,10 @@ inline hipError_t density_scatter(const GridGeom &G, int64_t N, int Bp1, PosView
     hipLaunchKernelGGL(k_dens_place, dim3(gb), dim3(256), 0, st, N, S.cell, S.off, S.fill, S.list);
     hipLaunchKernelGGL(k_dens_sort, dim3(gv), dim3(256), 0, st, nb, S.off, S.cnt, S.list);
     if (nb >= (int64_t)1024 * GATHER_BIG_BLOCKS)
-        hipLaunchKernelGGL(k_dens_gather<1024>, dim3((unsigned)((nb + 1023) / 1024)), dim3(1024), 0, st, G, Bp1,
+        hipLaunchKernelGGL((k_dens_gather<1024, true>), dim3((unsigned)((nb + 1023) / 1024)), dim3(1024), 0, st, G, Bp1,
                            S.off, S.cnt, S.list, S.brec, density, cleanup);
     else
-        hipLaunchKernelGGL(k_dens_gather<256>, dim3(gv), dim3(256), 0, st, G, Bp1, S.off, S.cnt, S.list, S.brec,
+        hipLaunchKernelGGL((k_dens_gather<256, false>), dim3(gv), dim3(256), 0, st, G, Bp1, S.off, S.cnt, S.list, S.brec,
                            density, cleanup);
     return hipGetLastError();
 }
