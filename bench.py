@@ -4,16 +4,23 @@ Headline workload (``value``): 512 independent replicas advanced together by
 the persistent kernel, chain lengths N ~ U{2000..50000}, 32^3..64^3 voxels of
 30 nm, per-replica chi ~ U[0,2], eps ~ U[-5,-1], mu ~ U[-2,1], Bernoulli(0.3)
 marks, Philox streams.  One *step* is one mc_sim sweep of every replica
-(all_moves: 161 proposals per replica) in one kernel launch.  The launch shape
-(wavefronts per replica) is chosen by chromo_amd.autotune after the warmup; all
-shapes produce the same chains.  The timed sweeps replay a captured HIP graph
-of one sweep (``--no-graph``: direct launches).
+(all_moves: 161 proposals per replica).
+
+The chains are first run ``--equilibrate`` sweeps (default 400) so that the
+move controllers reach their steady-state amplitudes (the mean move grows from
+~2.6 beads at the start to ~6.8); the timed window is the sweep range reported
+in ``config.sweeps_timed``.  After the warmup, chromo_amd.autotune times every
+pair (wavefronts per replica, sweeps per graph launch) and keeps the fastest;
+all settings produce the same chains.  The timed steps replay a captured HIP
+graph of that many sweeps (``--no-graph``: direct launches).
 
 Also reported: the plain path (one kernel launch per proposal) on the
 100,000-bead / 64^3 configuration, the ratio of the two, a dE / decision check
-of four replicas against the CPU oracle, the roofline of the dominant kernel
-(algorithmic HBM bytes per launch over its HIP-event-timed duration), and the
-CPU oracle timed on a bounded sample of the same replicas on the host cores.
+of four replicas against the CPU oracle, near-tie counters of the timed window,
+the roofline of the dominant kernel (algorithmic HBM bytes per launch over its
+HIP-event-timed duration), and the CPU oracle on the host cores: the same 512
+replicas from the same state over the same sweep window (lean and
+reference-faithful builds), and the 1,000-bead / 30^3 configuration.
 
 ``--adjoint`` instead times the energy + adjoint evaluation (E, dE/dr, dE/dtheta)
 on the 20,000-bead and 10^6-bead configurations and prints its own JSON line.
@@ -22,6 +29,7 @@ from __future__ import annotations
 
 import argparse
 import copy
+import dataclasses
 import json
 import os
 import sys
@@ -61,6 +69,10 @@ def timed(fn, n):
     return start.elapsed_time(stop) / 1000.0
 
 
+def _divisors(k, cands=(1, 2, 4, 5, 10, 20, 25, 40, 50)):
+    return [c for c in cands if c <= k and k % c == 0]
+
+
 def bench_batched(args):
     from chromo_amd import _lib as L
     from chromo_amd import workloads as W
@@ -70,33 +82,73 @@ def bench_batched(args):
                         list(range(len(systems))))
     log(f"batched: {len(systems)} replicas built in {time.time() - t0:.1f}s "
         f"(beads {sum(s.chain.N for s in systems)})")
+    done = 0
+    # steady state: the controllers raise the move amplitudes over the first few
+    # hundred sweeps; run them as part of preparing the workload
+    while done < args.equilibrate:
+        k = min(100, args.equilibrate - done)
+        dr.run(k, sync=False, waves=2)
+        done += k
+    dr.raise_on_error()
     for _ in range(args.warmup):
-        dr.run(1, sync=False, waves=1)
+        dr.run(1, sync=False, waves=2)
+    done += args.warmup
     dr.raise_on_error()
     if args.waves == "auto":
         from chromo_amd.autotune import autotune
-        tune = autotune(dr, sweeps=2)
-        log("autotune (s/sweep): " + ", ".join(f"W={w}: {t * 1e3:.3f} ms" for w, t in tune.items())
-            + f" -> W={dr.waves}")
+        spl = _divisors(args.steps) if args.sweeps_per_launch == "auto" else [int(args.sweeps_per_launch)]
+        c_before = int(dr.counter_array()[:, L.CNT_PROPOSALS].sum())
+        tune = autotune(dr, sweeps=20, sweeps_per_launch=spl)
+        # every candidate's graph ran sweeps (ordinary sweeps of the same chains)
+        done += sum(s_ + max(1, 20 // s_) * s_ for (_, s_) in tune)
+        log("autotune (ms/sweep): " + ", ".join(f"W={w} S={s_}: {t * 1e3:.3f}"
+                                               for (w, s_), t in tune.items())
+            + f" -> W={dr.waves}, S={dr.sweeps_per_launch}")
     else:
         tune = None
         dr.waves = int(args.waves)
-    c0 = dr.counter_array()
+        dr.sweeps_per_launch = 1 if args.sweeps_per_launch == "auto" else int(args.sweeps_per_launch)
+    spl = dr.sweeps_per_launch
+    if args.steps % spl:
+        raise SystemExit(f"--steps {args.steps} is not a multiple of {spl} sweeps per launch")
     if args.no_graph:
-        secs = timed(lambda: dr.run(1, sync=False), args.steps)
+        g = None
+        step = lambda: dr.run(spl, sync=False)
     else:
-        g = dr.capture(1)               # one sweep of every replica per graph launch
-        c0 = dr.counter_array()
-        secs = timed(g.replay, args.steps)
+        g = dr.capture(spl)             # spl sweeps of every replica per graph launch
+        step = g.replay
+    torch_sync()
+    snap = snapshot(dr, systems) if not args.no_cpu_baseline else None
+    c0 = dr.counter_array()
+    secs = timed(step, args.steps // spl)
     dr.raise_on_error()
     c1 = dr.counter_array()
     d = c1 - c0
     props = int(d[:, L.CNT_PROPOSALS].sum())
     nbytes = int(d[:, L.CNT_BYTES].sum())
     return dict(secs=secs, proposals=props, bytes=nbytes, accepted=int(d[:, L.CNT_ACCEPTED].sum()),
-                near_ties=int(d[:, L.CNT_NEAR_TIES].sum()), replicas=len(systems),
-                beads=int(sum(s.chain.N for s in systems)), systems=systems, waves=dr.waves,
-                tune=tune)
+                near_ties={"metropolis": int(d[:, L.CNT_NEAR_TIES].sum()),
+                           "voxel_floor": int(d[:, L.CNT_TIE_VOXEL].sum()),
+                           "over_vf_limit_voxels": int(d[:, L.CNT_TIE_HUGE].sum())},
+                replicas=len(systems), beads=int(sum(s.chain.N for s in systems)), systems=systems,
+                waves=dr.waves, spl=spl, tune=tune, first_sweep=done, snap=snap,
+                launches=args.steps // spl, mean_n=float(d[:, L.CNT_SUM_N].sum()) / max(1, props))
+
+
+def torch_sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def snapshot(dr, systems):
+    """Every replica's chain, binder states and controllers at the start of the
+    timed window (host copies), for the CPU leg."""
+    out = []
+    for i, s in enumerate(systems):
+        r, t3, t2 = dr.positions(i)
+        out.append((dataclasses.replace(s.chain, r=r, t3=t3, t2=t2, states=dr.binder_states(i)),
+                    dr.controllers(i)))
+    return out
 
 
 def bench_plain(args):
@@ -139,22 +191,68 @@ def dE_check(systems, sweeps=20, n=4, waves=1):
     return dict(proposals=total, max_abs_dE_err=worst, decision_mismatches=mism)
 
 
-def cpu_baseline(systems, target_s=15.0):
-    """The C oracle (lean build) on a bounded sample of the batched replicas."""
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """The host cores this job may use: the affinity set, capped by OMP_NUM_THREADS
+    where the machine sets it to the share of cores that goes with one GPU."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_baseline(b, steps, config1_sweeps=1000):
+    """The C oracle on the host cores, one replica per thread at a time:
+
+    * the same 512 replicas, from the same chains, binder states and controller
+      amplitudes as the GPU's timed window, over the same number of sweeps, in
+      the lean build (affected voxels only) and the reference-faithful build
+      (the reference's O(n_bins) reset and accept scans per field move);
+    * the 1,000-bead / 30^3 configuration, one replica per thread.
+    """
+    from chromo_amd import workloads as W
     from oracle import oracle as O
-    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
-                         else os.cpu_count() or 1))
-    sample = systems[:threads]
-    sims = [O.OracleSim(s.chain, s.field, rng_mode=O.RNG_PHILOX, seed=20240601, stream=i)
-            for i, s in enumerate(sample)]
-    ctrls = [copy.deepcopy(s.ctrls) for s in sample]
-    # calibrate: one sweep, then enough sweeps for ~target_s
-    p, sec = O.run_batch(sims, ctrls, 1, threads)
-    sweeps = int(max(1, min(2000, target_s / max(sec, 1e-3))))
-    p, sec = O.run_batch(sims, ctrls, sweeps, threads)
-    return dict(value=p / sec, unit="move-evals/s", cores=threads, kind="port",
-                sample=f"{len(sample)} batched replicas (one per thread), {sweeps} sweeps, "
-                       f"{p} proposals in {sec:.1f} s, oracle lean build")
+    threads = cpu_threads()
+    snap, systems = b["snap"], b["systems"]
+    out = {}
+    for kind, faithful in (("lean", False), ("faithful", True)):
+        t0 = time.time()
+        sims = [O.OracleSim(ch, s.field, rng_mode=O.RNG_PHILOX, seed=20240601, stream=i, faithful=faithful)
+                for i, ((ch, _), s) in enumerate(zip(snap, systems))]
+        ctrls = [copy.deepcopy(c) for _, c in snap]
+        log(f"cpu {kind}: {len(sims)} oracle replicas built in {time.time() - t0:.1f}s")
+        p, sec = O.run_batch(sims, ctrls, steps, threads)
+        out[kind] = dict(value=p / sec, proposals=p, seconds=sec)
+        log(f"cpu {kind}: {p} proposals in {sec:.2f}s -> {p / sec:.4g}/s")
+        del sims
+    s1 = W.config_plain_cpu()
+    c1 = {}
+    for kind, faithful in (("lean", False), ("faithful", True)):
+        sweeps = config1_sweeps if not faithful else max(1, config1_sweeps // 5)
+        sims = [O.OracleSim(s1.chain, s1.field, rng_mode=O.RNG_PHILOX, seed=1, stream=i, faithful=faithful)
+                for i in range(threads)]
+        ctrls = [copy.deepcopy(s1.ctrls) for _ in range(threads)]
+        p, sec = O.run_batch(sims, ctrls, sweeps, threads)
+        c1[kind] = dict(value=p / sec, per_core=p / sec / threads, sweeps=sweeps, proposals=p)
+        log(f"cpu config 1 {kind}: {p / sec:.4g}/s on {threads} threads")
+    first = b["first_sweep"]
+    return dict(value=out["lean"]["value"], unit="move-evals/s", cores=threads, kind="port",
+                cpu_model=cpu_model(),
+                sample=(f"all {len(snap)} batched replicas from the GPU's state at the start of its "
+                        f"timed window, sweeps {first}..{first + steps} ({out['lean']['proposals']} "
+                        f"proposals in {out['lean']['seconds']:.2f} s), oracle lean build, one replica "
+                        f"per thread at a time on {threads} threads"),
+                faithful=out["faithful"], config1={"workload": "1,000 beads, 30^3 voxels, B=2", **c1})
 
 
 def bench_adjoint(args):
@@ -193,7 +291,7 @@ def bench_adjoint(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--plain-steps", type=int, default=40)
@@ -203,6 +301,11 @@ def main():
                     help="launch each sweep directly instead of replaying a captured HIP graph")
     ap.add_argument("--waves", default="auto",
                     help="wavefronts per replica (1 or 2) or 'auto' to time each and keep the fastest")
+    ap.add_argument("--sweeps-per-launch", default="auto",
+                    help="sweeps per graph launch, or 'auto' (with --waves auto: time the divisors of "
+                         "--steps among 1, 2, 4, 5, 10, 20, ...)")
+    ap.add_argument("--equilibrate", type=int, default=400,
+                    help="sweeps run before the warmup so that the move amplitudes are at steady state")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 256 MiB device to device 5 times (calibrates rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE counters against a known byte count)")
@@ -244,7 +347,7 @@ def main():
     chk = None if args.no_check else dE_check(b["systems"], waves=b["waves"])
     if chk:
         log(f"check: {chk}")
-    cpu = None if args.no_cpu_baseline else cpu_baseline(b["systems"])
+    cpu = None if args.no_cpu_baseline else cpu_baseline(b, args.steps)
     if cpu:
         log(f"cpu: {cpu}")
 
@@ -254,7 +357,10 @@ def main():
     prof = ROOT / "profiles" / f"traffic_{kname}.json"
     if prof.exists():
         try:
-            traffic = json.loads(prof.read_text()).get("hbm_bytes_per_launch")
+            tj = json.loads(prof.read_text())
+            # measured on this workload's timed window at this many sweeps per launch
+            if int(tj.get("sweeps_per_launch", 1)) == b["spl"]:
+                traffic = tj.get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
     out = {
@@ -275,10 +381,14 @@ def main():
                         f"({b['beads']} beads), 32^3-64^3 voxels of 30 nm, B=2, all_moves + "
                         "SimpleControl, Philox; one step = one sweep (161 proposals per replica)",
             "replicas": b["replicas"],
+            "sweeps_timed": [b["first_sweep"], b["first_sweep"] + args.steps],
+            "mean_beads_per_move": b["mean_n"],
             "waves_per_replica": b["waves"],
+            "sweeps_per_launch": b["spl"],
             "parallelism": (f"replicas, one workgroup each; {b['waves']} wavefront(s) per replica "
-                            "evaluating proposals of a controller block at once"),
-            "autotune_ms_per_sweep": ({str(w): t * 1e3 for w, t in b["tune"].items()}
+                            "evaluating proposals of a controller block at once; "
+                            f"{b['spl']} sweep(s) per graph launch"),
+            "autotune_ms_per_sweep": ({f"W={w},S={s_}": t * 1e3 for (w, s_), t in b["tune"].items()}
                                       if b["tune"] else None),
         },
         "plain": {"value": plain_value, "unit": "move-evals/s",
@@ -291,8 +401,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_mc_run" if b["waves"] == 1 else f"k_mc_spec<{b['waves']}, 2>",
-                     "algorithmic_bytes_per_launch": b["bytes"] / args.steps},
+                     "algorithmic_bytes_per_launch": b["bytes"] / b["launches"],
+                     "launch_ms": 1000.0 * b["secs"] / b["launches"]},
         "cpu_baseline": cpu,
+        "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
+        "gpu_over_cpu_faithful": (value / cpu["faithful"]["value"]) if cpu else None,
     }
     print(json.dumps(out), flush=True)
 

@@ -28,8 +28,8 @@ RNG_MT_STATE, RNG_MT_INDEX = 64, 688
 RNG_PHILOX_KEY, RNG_PHILOX_CTR, RNG_PHILOX_BUF, RNG_PHILOX_IDX = 690, 692, 694, 698
 
 (CNT_PROPOSALS, CNT_ACCEPTED, CNT_SUM_N, CNT_SUM_BINS, CNT_ERROR, CNT_NEAR_TIES, CNT_BYTES,
- CNT_LOGGED) = range(8)
-CNT_WORDS = 8
+ CNT_LOGGED, CNT_TIE_VOXEL, CNT_TIE_HUGE) = range(10)
+CNT_WORDS = 10
 
 ERROR_TEXT = {
     1: "bead selection window larger than the polymer (ValueError in bead_selection)",
@@ -122,6 +122,8 @@ _SIGNATURES = {
     "chromo_field_dE_tape": (ctypes.c_int, [_c_void_p, _c_i32, _c_void_p, _c_i64, _c_void_p,
                                             _c_void_p, _c_i32, _c_void_p, _c_void_p, _c_void_p,
                                             _c_void_p, _c_void_p]),
+    "chromo_capped_exponential_tape": (ctypes.c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p,
+                                                      _c_void_p]),
     "chromo_energy_adjoint": (ctypes.c_int, [ctypes.POINTER(ChromoEnergyModel), _c_void_p,
                                              _c_void_p, _c_void_p, _c_void_p, _c_void_p,
                                              _c_void_p]),

@@ -29,6 +29,7 @@ constexpr int MAX_CTRL = 8;
 constexpr double RMAX = 2147483647.0;       // RAND_MAX as a double
 constexpr double E_HUGE_FIELD = 1E99;       // chromo/fields.pyx:32
 constexpr double E_HUGE_POLY = 1E25;        // chromo/polymers.pyx:35
+constexpr double TIE_VOXEL_EPS = 1E-11;     // voxel units: a floor that an ulp could flip
 constexpr int HKEY_EMPTY = -1;
 constexpr int HMIN_EMPTY = 0x7fffffff;
 
@@ -596,6 +597,15 @@ __device__ __forceinline__ double pymod(double a, double b)
 __device__ __forceinline__ int64_t imax64(int64_t a, int64_t b) { return a > b ? a : b; }
 __device__ __forceinline__ int64_t imin64(int64_t a, int64_t b) { return a < b ? a : b; }
 
+// The truncated exponential of one draw x (bead_selection.pyx:58-65), in the
+// reference's operation order.  Every device path draws bead windows through this
+// one function, and chromo_capped_exponential_tape exposes it for the exhaustive
+// comparison with glibc's log10 near every integer crossing.
+__device__ __forceinline__ int64_t cexp_trunc(int32_t x, int64_t window)
+{
+    return (int64_t)(-log10(((double)x / RMAX) + 0.00001) * (double)window * 0.45 + 1.0001);
+}
+
 // capped_exponential, bead_selection.pyx:19-66
 // The draw accepts r <= cap with probability above 0.99 for cap >= 1; cap < 1 can
 // never accept (the reference loops for ever) and is reported as ERR_SPIN.
@@ -608,8 +618,7 @@ __device__ __forceinline__ int64_t capped_exponential(WaveRng &R, int64_t window
     int64_t r;
     int it = 0;
     do {
-        r = (int64_t)(-log10(((double)rnd31(R) / RMAX) + 0.00001) * (double)window * 0.45 +
-                      1.0001);
+        r = cexp_trunc(rnd31(R), window);
         if (++it >= SPIN_LIMIT) {
             err = ERR_SPIN;
             return 0;
@@ -806,6 +815,11 @@ __device__ __forceinline__ void voxel_dE_terms(const chromo_replica *R, const do
     const double V = R->bead_vol, vf = R->vf_limit, chi = R->chi, vol_bin = R->vol_bin;
     double vf0 = r[0] * V;
     double vf1 = vf0 + (a[0] * V);
+    // over vf_limit before and after: the reference's running sum adds +1e99 vf1
+    // and -1e99 vf0 in CPython set order, which drops (vf1 == vf0) or rounds away
+    // the small terms summed before this voxel; the result depends on that order.
+    // Counted, one atomic per (rare) voxel event.
+    if (vf0 > vf && vf1 > vf) atomicAdd((unsigned long long *)&R->counters[CHROMO_CNT_TIE_HUGE], 1ull);
     if (vf1 > vf) NS += E_HUGE_FIELD * vf1;
     else NS += chi * (vol_bin / V) * (vf1 * vf1);
     if (vf0 > vf) NS -= E_HUGE_FIELD * vf0;
@@ -862,6 +876,11 @@ __device__ __forceinline__ void voxel_lower(const chromo_replica *R, const doubl
         int64_t ind = (int64_t)floor(q);
         ix[j] = (ind == -1) ? nn[j] - 1 : ind;
         w[j] = 1 - (q - (double)ind);
+        // within 1e-11 voxel of a voxel plane, an ulp of difference in the position
+        // (device sin / cos / acos / sqrt against glibc's) could change the floor:
+        // counted, one atomic per (rare) event
+        if (fabs(q - rint(q)) < TIE_VOXEL_EPS)
+            atomicAdd((unsigned long long *)&R->counters[CHROMO_CNT_TIE_VOXEL], 1ull);
     }
 }
 

@@ -484,6 +484,13 @@ int set_last_error(int code, const std::string &msg)
 }
 }  // namespace chromo
 
+// chromo_capped_exponential_tape: cexp_trunc of n (draw, window) pairs.
+__global__ void k_cexp_tape(const int32_t *draws, const int64_t *windows, int64_t n, int64_t *out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = cexp_trunc(draws[i], windows[i]);
+}
+
 extern "C" {
 
 int chromo_last_error(char *buf, size_t len)
@@ -700,6 +707,21 @@ int chromo_field_dE_tape(const chromo_replica *d_reps, int32_t rep, const int64_
     hipLaunchKernelGGL(k_field_tape, dim3(1), dim3(64), g_lds_bytes, st, d_reps, rep, d_inds, n,
                        d_r_trial, d_states_trial, state_change, d_dE, d_bins, d_delta, d_n_bins,
                        g_lds_bp1);
+    HIPC(launch_err());
+    return 0;
+}
+
+int chromo_capped_exponential_tape(const int32_t *d_draws, const int64_t *d_windows, int64_t n,
+                                   int64_t *d_out, void *stream)
+{
+    if (n < 0 || (n > 0 && (d_draws == nullptr || d_windows == nullptr || d_out == nullptr)))
+        return set_err(-1, "bad arguments");
+    if (n == 0) return 0;
+    const int threads = 256;
+    const int64_t blocks = (n + threads - 1) / threads;
+    if (blocks > 0x7fffffff) return set_err(-1, "n too large");
+    hipLaunchKernelGGL(k_cexp_tape, dim3((unsigned)blocks), dim3(threads), 0, (hipStream_t)stream, d_draws,
+                       d_windows, n, d_out);
     HIPC(launch_err());
     return 0;
 }

@@ -150,7 +150,7 @@ __device__ __forceinline__ int64_t cexp_once(int32_t x, int64_t window, int64_t 
         ok = false;
         return 0;
     }
-    const int64_t r = (int64_t)(-log10(((double)x / RMAX) + 0.00001) * (double)window * 0.45 + 1.0001);
+    const int64_t r = cexp_trunc(x, window);
     if (r > cap) ok = false;
     return r;
 }

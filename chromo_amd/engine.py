@@ -151,7 +151,8 @@ class DeviceReplicas:
     CTRL_SLOTS = 8
     SEG_MAX = 32       # beads per move of the one-wavefront evaluator (chromo_seg.hpp)
     WAVE_CHOICES = (1, 2)
-    waves = 1          # wavefronts per replica used by run(); see chromo_amd.autotune
+    waves = 1              # wavefronts per replica used by run(); see chromo_amd.autotune
+    sweeps_per_launch = 1  # sweeps per captured graph launch chosen by chromo_amd.autotune
 
     def __init__(self, chains: Sequence[ChainSpec], fields: Sequence[FieldSpec],
                  ctrls: Sequence[Sequence[CtrlSpec]], *, rng_mode: int = L.RNG_GLIBC,

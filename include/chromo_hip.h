@@ -89,7 +89,16 @@ enum chromo_counter {
     CHROMO_CNT_NEAR_TIES = 5,   /* |u - exp(-dE)| < 1e-12 */
     CHROMO_CNT_BYTES = 6,       /* algorithmic HBM bytes of all proposals */
     CHROMO_CNT_LOGGED = 7,      /* records written to log */
-    CHROMO_CNT_WORDS = 8
+    CHROMO_CNT_TIE_VOXEL = 8,   /* bead coordinates within 1e-11 voxel of a voxel plane, where an
+                                   ulp of difference between device and glibc sin/cos/acos/sqrt
+                                   could change the voxel floor (every evaluation, speculative
+                                   ones included) */
+    CHROMO_CNT_TIE_HUGE = 9,    /* touched voxels over vf_limit both before and after a move: the
+                                   reference's running sum of +-1e99 volume-fraction terms then
+                                   drops or rounds away the terms summed before them in CPython
+                                   set order, so its dE depends on that order (every evaluation,
+                                   speculative ones included) */
+    CHROMO_CNT_WORDS = 10
 };
 
 /* Values of counters[CHROMO_CNT_ERROR].  A replica whose counter is non-zero is
@@ -216,6 +225,13 @@ int chromo_field_dE_tape(const chromo_replica *d_reps, int32_t rep, const int64_
                          int64_t n, const double *d_r_trial, const int8_t *d_states_trial,
                          int32_t state_change, double *d_dE, int64_t *d_bins,
                          double *d_delta, int64_t *d_n_bins, void *stream);
+
+/* (int)(-log10(x / RAND_MAX + 1e-5) * window * 0.45 + 1.0001) for n draws x and
+ * windows, with the device arithmetic every bead-window draw uses
+ * (capped_exponential, chromo/util/bead_selection.pyx:58-65).  Parity testing:
+ * the draws next to each integer crossing are compared with glibc's log10. */
+int chromo_capped_exponential_tape(const int32_t *d_draws, const int64_t *d_windows, int64_t n,
+                                   int64_t *d_out, void *stream);
 
 /* ---- absolute energies and adjoint -------------------------------------- */
 

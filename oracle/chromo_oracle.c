@@ -637,11 +637,15 @@ void or_transform(const double T[16], const double *r, const double *t3, const d
 /* ------------------------------------------------------------------------- */
 
 /* capped_exponential, bead_selection.pyx:19-66 */
+static inline int64_t cexp_value(int32_t x, int64_t window)
+{
+    return (int64_t)(-log10(((double)x / RAND_MAX_I) + 0.00001) * window * 0.45 + 1.0001);
+}
+
 static int64_t capped_exponential(rng_t *g, int64_t window, int64_t cap)
 {
-    int64_t r = (int64_t)(-log10(((double)rng_rand(g) / RAND_MAX_I) + 0.00001) * window * 0.45 + 1.0001);
-    while (r > cap)
-        r = (int64_t)(-log10(((double)rng_rand(g) / RAND_MAX_I) + 0.00001) * window * 0.45 + 1.0001);
+    int64_t r = cexp_value(rng_rand(g), window);
+    while (r > cap) r = cexp_value(rng_rand(g), window);
     return r;
 }
 
@@ -1386,8 +1390,10 @@ int or_mc_step(or_sim *s, or_ctrl *c, or_log *log)
         log->move = move;
         log->accepted = acc;
         log->n = n;
+        /* a tangent rotation's picks are not a window: its record holds the
+         * first picked bead, as the device's */
         log->ind0 = inds[0];
-        log->indf = inds[n - 1] + 1;
+        log->indf = move == OR_TANGENT ? inds[0] + 1 : inds[n - 1] + 1;
         log->dE_poly = dEp;
         log->dE_field = dEf;
         log->u = u;
@@ -1549,6 +1555,21 @@ static double py_round2(double x)
     long double z = nearbyintl(y);
     return (double)z / 100.0;
 }
+
+/* The truncated exponential of capped_exponential (bead_selection.pyx:58-65) for
+ * given draws and windows, glibc log10: the reference side of the exhaustive
+ * comparison with the device arithmetic near every integer crossing. */
+void or_cexp_trunc(const int32_t *draws, const int64_t *windows, int64_t n, int64_t *out)
+{
+    for (int64_t i = 0; i < n; ++i) out[i] = cexp_value(draws[i], windows[i]);
+}
+
+/* Python's round(x, 2), exposed for its known-answer test against CPython. */
+double or_py_round2(double x) { return py_round2(x); }
+
+/* mu_adjust_factor of the next tape evaluation (mc_sim sets it from its
+ * argument, chromo/mc/mc_sim.pyx:97). */
+void or_set_mu_adjust(or_sim *s, double f) { s->mu_adjust_factor = f; }
 
 double or_field_E(or_sim *s)
 {

@@ -102,6 +102,12 @@ double or_binding_dE_tape(or_sim *s, int64_t ind0, int64_t indf, const int64_t *
 double or_elastic_E(const or_sim *s);
 double or_field_E(or_sim *s);
 
+/* capped_exponential's truncated value for given draws and windows (glibc log10). */
+void or_cexp_trunc(const int32_t *draws, const int64_t *windows, int64_t n, int64_t *out);
+/* Python round(x, 2) as compute_E uses it (fields.pyx:2282). */
+double or_py_round2(double x);
+/* Sets mu_adjust_factor for tape evaluations (mc_sim sets it per call). */
+void or_set_mu_adjust(or_sim *s, double f);
 /* RNG primitives, exposed for known-answer tests. */
 void or_glibc_stream(uint32_t seed, int32_t *out, int64_t n);
 void or_mt_randint_stream(uint32_t seed, int64_t high, int64_t *out, int64_t n);

@@ -105,6 +105,10 @@ def lib():
         h.or_elastic_E.argtypes = [V]
         h.or_field_E.restype = ctypes.c_double
         h.or_field_E.argtypes = [V]
+        h.or_cexp_trunc.argtypes = [ctypes.POINTER(ctypes.c_int32), _ip, ctypes.c_int64, _ip]
+        h.or_py_round2.restype = ctypes.c_double
+        h.or_py_round2.argtypes = [ctypes.c_double]
+        h.or_set_mu_adjust.argtypes = [V, ctypes.c_double]
         h.or_glibc_stream.argtypes = [ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64]
         h.or_mt_randint_stream.argtypes = [ctypes.c_uint32, ctypes.c_int64, _ip, ctypes.c_int64]
         h.or_philox4x32_10.argtypes = [ctypes.POINTER(ctypes.c_uint32)] * 3
@@ -253,11 +257,26 @@ class OracleSim:
     def binding_dE_tape(self, ind0, indf, states_trial):
         return self.h.or_binding_dE_tape(self.p, int(ind0), int(indf), _iptr(_i(states_trial)))
 
+    def set_mu_adjust(self, f):
+        self.h.or_set_mu_adjust(self.p, float(f))
+
     def elastic_E(self):
         return self.h.or_elastic_E(self.p)
 
     def field_E(self):
         return self.h.or_field_E(self.p)
+
+
+def cexp_trunc(draws, windows):
+    d = np.ascontiguousarray(draws, dtype=np.int32)
+    w = _i(windows)
+    out = np.zeros(len(d), dtype=np.int64)
+    lib().or_cexp_trunc(d.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _iptr(w), len(d), _iptr(out))
+    return out
+
+
+def py_round2(x):
+    return lib().or_py_round2(float(x))
 
 
 def glibc_stream(seed, n):
