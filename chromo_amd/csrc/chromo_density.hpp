@@ -117,20 +117,54 @@ __global__ void k_dens_place(int64_t N, const int32_t *cell, const int32_t *off,
     list[pos] = (int32_t)b;
 }
 
+// Puts each bucket (filled in atomic order by k_dens_place) in bead order.  Small
+// buckets, the usual case, by insertion sort; large ones (dense voxels: compact
+// or collapsed chains, coarse grids) by heapsort, so the cost stays k log k.
+__device__ __forceinline__ void sift_down(int32_t *a, int32_t i, int32_t n)
+{
+    const int32_t x = a[i];
+    for (;;) {
+        int32_t c = 2 * i + 1;
+        if (c >= n) break;
+        if (c + 1 < n && a[c + 1] > a[c]) ++c;
+        if (a[c] <= x) break;
+        a[i] = a[c];
+        i = c;
+    }
+    a[i] = x;
+}
+
 __global__ void k_dens_sort(int64_t nbins, const int32_t *off, const int32_t *cnt, int32_t *list)
 {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nbins) return;
-    int32_t s0 = off[c], s1 = s0 + cnt[c];
-    for (int32_t a = s0 + 1; a < s1; ++a) {
-        int32_t x = list[a];
-        int32_t q = a - 1;
-        while (q >= s0 && list[q] > x) {
-            list[q + 1] = list[q];
-            --q;
+    const int32_t s0 = off[c], k = cnt[c];
+    int32_t *a = list + s0;
+    if (k <= 32) {
+        for (int32_t p = 1; p < k; ++p) {
+            const int32_t x = a[p];
+            int32_t q = p - 1;
+            while (q >= 0 && a[q] > x) {
+                a[q + 1] = a[q];
+                --q;
+            }
+            a[q + 1] = x;
         }
-        list[q + 1] = x;
+        return;
     }
+    for (int32_t i = k / 2 - 1; i >= 0; --i) sift_down(a, i, k);
+    for (int32_t e = k - 1; e > 0; --e) {
+        const int32_t t = a[0];
+        a[0] = a[e];
+        a[e] = t;
+        sift_down(a, 0, e);
+    }
+}
+
+__global__ void k_fill_i32(int32_t *p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 // The buckets that reach voxel v: cell[l] holds the beads of which v is corner l;
@@ -284,6 +318,34 @@ struct DensityScratch {
     int64_t maxN = 0, maxBins = 0;
 };
 
+constexpr size_t SCRATCH_ALIGN = 256;
+
+inline size_t align_scratch(size_t o) { return (o + SCRATCH_ALIGN - 1) / SCRATCH_ALIGN * SCRATCH_ALIGN; }
+
+// The scatter's scratch carved from one buffer at base (nullptr: sizes only);
+// returns the bytes used, a multiple of SCRATCH_ALIGN.
+inline size_t density_scratch_carve(DensityScratch &S, char *base, int64_t maxN, int64_t maxBins)
+{
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        char *p = base ? base + o : nullptr;
+        o = align_scratch(o + bytes);
+        return p;
+    };
+    S.maxN = maxN;
+    S.maxBins = maxBins;
+    S.cell = (int32_t *)take(sizeof(int32_t) * maxN);
+    S.list = (int32_t *)take(sizeof(int32_t) * maxN);
+    S.brec = (BeadRec *)take(sizeof(BeadRec) * maxN);
+    S.cnt = (int32_t *)take(sizeof(int32_t) * maxBins);
+    S.off = (int32_t *)take(sizeof(int32_t) * maxBins);
+    S.fill = (int32_t *)take(sizeof(int32_t) * maxBins);
+    S.bsum = (int32_t *)take(sizeof(int32_t) * 2048);
+    S.boff = (int32_t *)take(sizeof(int32_t) * 2048);
+    S.bsum2 = (int32_t *)take(sizeof(int32_t) * 2);
+    return o;
+}
+
 inline hipError_t density_scratch_alloc(DensityScratch &S, int64_t maxN, int64_t maxBins,
                                         hipStream_t st)
 {
@@ -322,10 +384,13 @@ inline hipError_t density_scatter(const GridGeom &G, int64_t N, int Bp1, PosView
 {
     const int64_t nb = G.nx * G.ny * G.nz;
     hipError_t e;
-    if ((e = hipMemsetAsync(S.cnt, 0, sizeof(int32_t) * nb, st)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(S.fill, 0, sizeof(int32_t) * nb, st)) != hipSuccess) return e;
     unsigned gb = (unsigned)((N + 255) / 256);
     unsigned gv = (unsigned)((nb + 255) / 256);
+    // cleared by a kernel rather than hipMemsetAsync: a caller may capture this
+    // sequence in a HIP graph
+    hipLaunchKernelGGL(k_fill_i32, dim3(gv), dim3(256), 0, st, S.cnt, nb);
+    hipLaunchKernelGGL(k_fill_i32, dim3(gv), dim3(256), 0, st, S.fill, nb);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_dens_cells, dim3(gb), dim3(256), 0, st, G, N, P, Bp1 - 1, states, S.cell, S.brec,
                        S.cnt);
     int64_t nbl = (nb + 2047) / 2048;

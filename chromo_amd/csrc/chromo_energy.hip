@@ -521,6 +521,8 @@ int make_model(const chromo_energy_model *m, EModel &M)
     if (m == nullptr) return eset(-1, "model is NULL");
     if (m->B < 1 || m->B > CHROMO_MAX_BINDERS) return eset(-1, "B outside 1..4");
     if (m->N < 2) return eset(-1, "N < 2");
+    // the scatter stores bead indices as int32 and uses 0x7fffffff as its empty marker
+    if (m->N >= 0x7fffffffLL) return eset(-1, "N must be below 2^31 - 1");
     if (n_theta(m->B) > MAXT || 2 + 2 * m->B > MAXT) return eset(-1, "too many terms");
     M.N = m->N;
     M.B = m->B;
@@ -553,6 +555,71 @@ int make_model(const chromo_energy_model *m, EModel &M)
     return 0;
 }
 
+// Device buffers of one energy evaluation, carved from one allocation (or from
+// the file-scope scratch array of chromo_energy_adjoint_noalloc).
+struct EnergyBufs {
+    DensityScratch S;
+    double *rho = nullptr, *dEdrho = nullptr, *vpart = nullptr, *bpart = nullptr;
+    int gv = 0, gb = 0;
+};
+
+// Carves the buffers of an evaluation of M from base (nullptr: sizes only) and
+// returns the bytes used.  rho_given: the density goes to the caller's array.
+size_t energy_carve(const EModel &M, bool grad, bool rho_given, char *base, EnergyBufs &Bf)
+{
+    const int64_t nb = M.G.nx * M.G.ny * M.G.nz;
+    const int Bp1 = M.B + 1;
+    size_t o = density_scratch_carve(Bf.S, base, M.N, nb);
+    auto take = [&](size_t bytes) {
+        double *p = base ? (double *)(base + o) : nullptr;
+        o = align_scratch(o + bytes);
+        return p;
+    };
+    Bf.gv = grid_blocks(nb);
+    Bf.gb = grid_blocks(M.N);
+    Bf.rho = rho_given ? nullptr : take(sizeof(double) * nb * Bp1);
+    Bf.dEdrho = grad ? take(sizeof(double) * nb * Bp1) : nullptr;
+    Bf.vpart = take(sizeof(double) * MAXT * Bf.gv);
+    Bf.bpart = take(sizeof(double) * MAXT * Bf.gb);
+    return o;
+}
+
+// One allocation (stream-ordered) freed on every return path.
+struct StreamBuffer {
+    char *p = nullptr;
+    hipStream_t st = nullptr;
+    ~StreamBuffer()
+    {
+        if (p) (void)hipFreeAsync(p, st);
+    }
+};
+
+// E, dE/dr and dE/dtheta of M at (r, theta) with the buffers Bf.
+int energy_adjoint_run(const EModel &M, const double *d_r, const double *d_theta, double *d_E, double *d_grad_r,
+                       double *d_grad_theta, EnergyBufs &Bf, hipStream_t st)
+{
+    const int B = M.B, Bp1 = B + 1;
+    PosView P{d_r, 1, 3};
+    hipError_t e = density_scatter(M.G, M.N, Bp1, P, M.states, Bf.rho, 0, Bf.S, st);
+    if (e != hipSuccess) return eset(-2, std::string("density scatter: ") + hipGetErrorString(e));
+    const int ntv = B + B * (B - 1) / 2 + 2, ntb = 2 + 2 * B;
+    const int gv = Bf.gv, gb = Bf.gb;
+    switch (B) {
+    case 1: launch_energy_terms<1>(M, d_theta, Bf.rho, Bf.dEdrho, Bf.vpart, gv, d_r, Bf.S, d_grad_r, Bf.bpart, gb, st); break;
+    case 2: launch_energy_terms<2>(M, d_theta, Bf.rho, Bf.dEdrho, Bf.vpart, gv, d_r, Bf.S, d_grad_r, Bf.bpart, gb, st); break;
+    case 3: launch_energy_terms<3>(M, d_theta, Bf.rho, Bf.dEdrho, Bf.vpart, gv, d_r, Bf.S, d_grad_r, Bf.bpart, gb, st); break;
+    default: launch_energy_terms<4>(M, d_theta, Bf.rho, Bf.dEdrho, Bf.vpart, gv, d_r, Bf.S, d_grad_r, Bf.bpart, gb, st); break;
+    }
+    switch (B) {
+    case 1: launch_energy_final<1>(M, d_theta, Bf.vpart, gv, ntv, Bf.bpart, gb, ntb, d_E, d_grad_theta, st); break;
+    case 2: launch_energy_final<2>(M, d_theta, Bf.vpart, gv, ntv, Bf.bpart, gb, ntb, d_E, d_grad_theta, st); break;
+    case 3: launch_energy_final<3>(M, d_theta, Bf.vpart, gv, ntv, Bf.bpart, gb, ntb, d_E, d_grad_theta, st); break;
+    default: launch_energy_final<4>(M, d_theta, Bf.vpart, gv, ntv, Bf.bpart, gb, ntb, d_E, d_grad_theta, st); break;
+    }
+    HIPE(hipGetLastError());
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -565,39 +632,13 @@ int chromo_energy_adjoint(const chromo_energy_model *model, const double *d_r, c
     if (rc) return rc;
     if (!d_r || !d_theta || !d_E) return eset(-1, "NULL argument");
     hipStream_t st = (hipStream_t)stream;
-    const int B = M.B, Bp1 = B + 1;
-    const int64_t nb = M.G.nx * M.G.ny * M.G.nz;
-    double *rho = nullptr, *dEdrho = nullptr, *vpart = nullptr, *bpart = nullptr;
-    DensityScratch S;
-    HIPE(density_scratch_alloc(S, M.N, nb, st));
-    HIPE(hipMallocAsync((void **)&rho, sizeof(double) * nb * Bp1, st));
-    if (d_grad_r) HIPE(hipMallocAsync((void **)&dEdrho, sizeof(double) * nb * Bp1, st));
-    const int gv = grid_blocks(nb), gb = grid_blocks(M.N);
-    HIPE(hipMallocAsync((void **)&vpart, sizeof(double) * MAXT * gv, st));
-    HIPE(hipMallocAsync((void **)&bpart, sizeof(double) * MAXT * gb, st));
-    PosView P{d_r, 1, 3};
-    hipError_t e = density_scatter(M.G, M.N, Bp1, P, M.states, rho, 0, S, st);
-    if (e != hipSuccess) return eset(-2, std::string("density scatter: ") + hipGetErrorString(e));
-    const int ntv = B + B * (B - 1) / 2 + 2, ntb = 2 + 2 * B;
-    switch (B) {
-    case 1: launch_energy_terms<1>(M, d_theta, rho, dEdrho, vpart, gv, d_r, S, d_grad_r, bpart, gb, st); break;
-    case 2: launch_energy_terms<2>(M, d_theta, rho, dEdrho, vpart, gv, d_r, S, d_grad_r, bpart, gb, st); break;
-    case 3: launch_energy_terms<3>(M, d_theta, rho, dEdrho, vpart, gv, d_r, S, d_grad_r, bpart, gb, st); break;
-    default: launch_energy_terms<4>(M, d_theta, rho, dEdrho, vpart, gv, d_r, S, d_grad_r, bpart, gb, st); break;
-    }
-    switch (B) {
-    case 1: launch_energy_final<1>(M, d_theta, vpart, gv, ntv, bpart, gb, ntb, d_E, d_grad_theta, st); break;
-    case 2: launch_energy_final<2>(M, d_theta, vpart, gv, ntv, bpart, gb, ntb, d_E, d_grad_theta, st); break;
-    case 3: launch_energy_final<3>(M, d_theta, vpart, gv, ntv, bpart, gb, ntb, d_E, d_grad_theta, st); break;
-    default: launch_energy_final<4>(M, d_theta, vpart, gv, ntv, bpart, gb, ntb, d_E, d_grad_theta, st); break;
-    }
-    HIPE(hipGetLastError());
-    density_scratch_free(S, st);
-    HIPE(hipFreeAsync(rho, st));
-    if (dEdrho) HIPE(hipFreeAsync(dEdrho, st));
-    HIPE(hipFreeAsync(vpart, st));
-    HIPE(hipFreeAsync(bpart, st));
-    return 0;
+    EnergyBufs Bf;
+    const size_t bytes = energy_carve(M, d_grad_r != nullptr, false, nullptr, Bf);
+    StreamBuffer buf;
+    buf.st = st;
+    HIPE(hipMallocAsync((void **)&buf.p, bytes, st));
+    energy_carve(M, d_grad_r != nullptr, false, buf.p, Bf);
+    return energy_adjoint_run(M, d_r, d_theta, d_E, d_grad_r, d_grad_theta, Bf, st);
 }
 
 int chromo_energy_reference(const chromo_energy_model *model, const double *d_pos, int32_t soa,
@@ -610,28 +651,24 @@ int chromo_energy_reference(const chromo_energy_model *model, const double *d_po
     if (!d_pos || !d_pref || !d_intra || !d_out) return eset(-1, "NULL argument");
     hipStream_t st = (hipStream_t)stream;
     const int B = M.B, Bp1 = B + 1;
-    const int64_t nb = M.G.nx * M.G.ny * M.G.nz;
-    double *rho = d_density_out, *vpart = nullptr, *bpart = nullptr;
-    DensityScratch S;
-    HIPE(density_scratch_alloc(S, M.N, nb, st));
-    if (!rho) HIPE(hipMallocAsync((void **)&rho, sizeof(double) * nb * Bp1, st));
-    const int gv = grid_blocks(nb), gb = grid_blocks(M.N);
-    HIPE(hipMallocAsync((void **)&vpart, sizeof(double) * MAXT * gv, st));
-    HIPE(hipMallocAsync((void **)&bpart, sizeof(double) * MAXT * gb, st));
+    EnergyBufs Bf;
+    const size_t bytes = energy_carve(M, false, d_density_out != nullptr, nullptr, Bf);
+    StreamBuffer buf;
+    buf.st = st;
+    HIPE(hipMallocAsync((void **)&buf.p, bytes, st));
+    energy_carve(M, false, d_density_out != nullptr, buf.p, Bf);
+    double *rho = d_density_out ? d_density_out : Bf.rho;
+    const int gv = Bf.gv, gb = Bf.gb;
     // UniformDensityField.update_all_densities: bead-order scatter without the 1e-18 clean-up
     const int64_t sc = soa ? M.N : 1, sbd = soa ? 1 : 9;
     PosView P{d_pos, sc, sbd};
-    hipError_t e = density_scatter(M.G, M.N, Bp1, P, M.states, rho, 0, S, st);
+    hipError_t e = density_scatter(M.G, M.N, Bp1, P, M.states, rho, 0, Bf.S, st);
     if (e != hipSuccess) return eset(-2, std::string("density scatter: ") + hipGetErrorString(e));
-    hipLaunchKernelGGL(k_ref_voxel, dim3(gv), dim3(EB), 0, st, M, d_pref, chi, rho, vpart, B + 1);
-    hipLaunchKernelGGL(k_ref_bead, dim3(gb), dim3(EB), 0, st, M, d_pos, sc, sbd, bpart, B + 1);
-    hipLaunchKernelGGL(k_ref_final, dim3(1), dim3(FINAL_THREADS), 0, st, M, d_pref, d_intra, vpart, gv, bpart, gb,
-                       d_out);
+    hipLaunchKernelGGL(k_ref_voxel, dim3(gv), dim3(EB), 0, st, M, d_pref, chi, rho, Bf.vpart, B + 1);
+    hipLaunchKernelGGL(k_ref_bead, dim3(gb), dim3(EB), 0, st, M, d_pos, sc, sbd, Bf.bpart, B + 1);
+    hipLaunchKernelGGL(k_ref_final, dim3(1), dim3(FINAL_THREADS), 0, st, M, d_pref, d_intra, Bf.vpart, gv, Bf.bpart,
+                       gb, d_out);
     HIPE(hipGetLastError());
-    density_scratch_free(S, st);
-    if (!d_density_out) HIPE(hipFreeAsync(rho, st));
-    HIPE(hipFreeAsync(vpart, st));
-    HIPE(hipFreeAsync(bpart, st));
     return 0;
 }
 

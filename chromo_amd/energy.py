@@ -65,6 +65,13 @@ class EnergyModel:
         if not 1 <= B <= L.MAX_BINDERS:
             raise NotImplementedError(f"binder count {B} outside 1..{L.MAX_BINDERS}")
         self.N, self.B = N, B
+        if N >= 2 ** 31 - 1:
+            raise NotImplementedError("chains of 2^31 - 1 beads or more")
+        st, mk = np.asarray(chain.states), np.asarray(chain.marks)
+        if st.size and (st.min() < 0 or st.max() > L.MAX_SITES):
+            raise ValueError(f"binding states outside 0..{L.MAX_SITES}")
+        if mk.size and (mk.min() < 0 or mk.max() >= L.MAX_MARK):
+            raise ValueError(f"chemical_mods outside 0..{L.MAX_MARK - 1}")
         self.t3 = torch.as_tensor(np.ascontiguousarray(chain.t3, dtype=np.float64), device=dev)
         self.states = torch.as_tensor(np.ascontiguousarray(np.asarray(chain.states).T, dtype=np.int8), device=dev)
         self.marks = torch.as_tensor(np.ascontiguousarray(np.asarray(chain.marks).T, dtype=np.int8), device=dev)

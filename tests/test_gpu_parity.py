@@ -326,3 +326,14 @@ def test_batched_equals_single(oracle):
             assert np.array_equal(a, b)
         assert np.array_equal(together.binder_states(i), alone.binder_states(0))
         assert np.array_equal(together.density_of(i), alone.density_of(0))
+
+
+def test_density_init_many_beads_per_voxel(oracle):
+    """10^5 beads on a 4^3 grid (about 1,600 beads in every bucket): the buckets
+    are put in bead order by heapsort, and the scatter stays bitwise the
+    reference's bead-order loop."""
+    say("density_init 1e5 beads on 4^3")
+    s = W.make_system(100000, 2, 4, seed=7)
+    s.chain.states = np.random.RandomState(7).randint(0, 3, size=(100000, 2))
+    dr = _dev([s])
+    assert np.array_equal(dr.density_of(0), oracle.OracleSim(s.chain, s.field).density())
