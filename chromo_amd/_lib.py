@@ -127,6 +127,9 @@ _SIGNATURES = {
     "chromo_energy_adjoint": (ctypes.c_int, [ctypes.POINTER(ChromoEnergyModel), _c_void_p,
                                              _c_void_p, _c_void_p, _c_void_p, _c_void_p,
                                              _c_void_p]),
+    "chromo_energy_adjoint_noalloc": (ctypes.c_int, [ctypes.POINTER(ChromoEnergyModel), _c_void_p,
+                                                     _c_void_p, _c_void_p, _c_void_p, _c_void_p,
+                                                     _c_void_p]),
     "chromo_energy_reference": (ctypes.c_int, [ctypes.POINTER(ChromoEnergyModel), _c_void_p,
                                                _c_i32, _c_void_p, _c_void_p, ctypes.c_double,
                                                _c_void_p, _c_void_p, _c_void_p]),

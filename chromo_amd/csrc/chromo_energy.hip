@@ -27,6 +27,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <mutex>
 #include <string>
 
 #include "chromo_density.hpp"
@@ -620,6 +621,36 @@ int energy_adjoint_run(const EModel &M, const double *d_r, const double *d_theta
     return 0;
 }
 
+// ---- the allocation-free evaluation -----------------------------------------
+
+// Device memory of chromo_energy_adjoint_noalloc: one array per GPU, held until
+// the process ends.  Host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (the array's name is a
+// host-side stand-in, not a device address).
+constexpr size_t NOALLOC_BYTES = size_t(32) << 20;
+__device__ __align__(256) char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+char *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The scratch array of the current device, looked up once per device.
+int noalloc_scratch(char **out)
+{
+    int dev = 0;
+    HIPE(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return eset(-1, "device index outside 0..63");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        HIPE(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return eset(-2, "hipGetSymbolAddress gave no address for the scratch array");
+        g_noalloc_ptr[dev] = (char *)p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -639,6 +670,25 @@ int chromo_energy_adjoint(const chromo_energy_model *model, const double *d_r, c
     HIPE(hipMallocAsync((void **)&buf.p, bytes, st));
     energy_carve(M, d_grad_r != nullptr, false, buf.p, Bf);
     return energy_adjoint_run(M, d_r, d_theta, d_E, d_grad_r, d_grad_theta, Bf, st);
+}
+
+int chromo_energy_adjoint_noalloc(const chromo_energy_model *model, const double *d_r, const double *d_theta,
+                                  double *d_E, double *d_grad_r, double *d_grad_theta, void *stream)
+{
+    EModel M;
+    int rc = make_model(model, M);
+    if (rc) return rc;
+    if (!d_r || !d_theta || !d_E) return eset(-1, "NULL argument");
+    EnergyBufs Bf;
+    const size_t bytes = energy_carve(M, d_grad_r != nullptr, false, nullptr, Bf);
+    if (bytes > NOALLOC_BYTES)
+        return eset(-1, "the evaluation needs " + std::to_string(bytes) + " bytes of scratch; the allocation-free "
+                        "entry point holds " + std::to_string(NOALLOC_BYTES) + " (use chromo_energy_adjoint)");
+    char *base = nullptr;
+    rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    energy_carve(M, d_grad_r != nullptr, false, base, Bf);
+    return energy_adjoint_run(M, d_r, d_theta, d_E, d_grad_r, d_grad_theta, Bf, (hipStream_t)stream);
 }
 
 int chromo_energy_reference(const chromo_energy_model *model, const double *d_pos, int32_t soa,

@@ -264,6 +264,21 @@ int chromo_energy_adjoint(const chromo_energy_model *model, const double *d_r,
                           const double *d_theta, double *d_E, double *d_grad_r,
                           double *d_grad_theta, void *stream);
 
+/* No counterpart in the reference.  chromo_energy_adjoint for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  Same arguments, same results bit for bit.
+ * Its scratch (the density scatter's buckets, the voxel densities and their
+ * adjoint, the block partials) lives in one file-scope device array of 32 MiB
+ * per GPU, held until the process ends; it calls no hipMalloc / hipFree.  An
+ * evaluation that needs more is rejected (-1, the bytes it needs in the message):
+ * 40 B per bead plus 12 + 8 (B+1) B per voxel (12 + 16 (B+1) with gradients),
+ * e.g. 20,000 beads and 48^3 voxels with B = 3 take 9 MiB.  Every call on one GPU shares that
+ * array and it keeps the last call's contents: calls that overlap on one GPU
+ * (two streams, two host threads) race. */
+int chromo_energy_adjoint_noalloc(const chromo_energy_model *model, const double *d_r,
+                                  const double *d_theta, double *d_E, double *d_grad_r,
+                                  double *d_grad_theta, void *stream);
+
 /* SSWLC.compute_E (chromo/polymers.pyx:1348-1381) and
  * UniformDensityField.compute_E (chromo/fields.pyx:1939-2286): d_out[0] =
  * elastic, d_out[1] = field (chi phi (1 - phi), round(phi, 2) > vf_limit, no
