@@ -83,6 +83,7 @@ class ChromoReplica(ctypes.Structure):
         ("n_mark_levels", ctypes.c_int32), ("confine_type", ctypes.c_int32),
         ("confine_length", ctypes.c_double),
         ("F", ctypes.c_double * (MAX_BINDERS * MAX_MARK * (MAX_SITES + 1))),
+        ("access_vol", ctypes.c_void_p),
     ]
 
 
@@ -122,6 +123,8 @@ _SIGNATURES = {
     "chromo_field_dE_tape": (ctypes.c_int, [_c_void_p, _c_i32, _c_void_p, _c_i64, _c_void_p,
                                             _c_void_p, _c_i32, _c_void_p, _c_void_p, _c_void_p,
                                             _c_void_p, _c_void_p]),
+    "chromo_checkpoint": (ctypes.c_int, [_c_void_p, _c_i32, ctypes.c_char_p, _c_void_p]),
+    "chromo_restore": (ctypes.c_int, [_c_void_p, _c_i32, ctypes.c_char_p, _c_void_p]),
     "chromo_capped_exponential_tape": (ctypes.c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p,
                                                       _c_void_p]),
     "chromo_energy_adjoint": (ctypes.c_int, [ctypes.POINTER(ChromoEnergyModel), _c_void_p,

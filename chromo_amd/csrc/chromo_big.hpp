@@ -125,6 +125,7 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
     const int nx = (int)R->nx, ny = (int)R->ny, nz = (int)R->nz;
     const bool degen = nx == 1 || ny == 1 || nz == 1;
     const double vol_bin = R->vol_bin;
+    const double *avol = R->access_vol;
     const int ct = R->confine_type;
     const double cl = R->confine_length;
     res.overflow = 0;
@@ -230,8 +231,8 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
                 const int4 pr = S.prec[p];
                 const double *wq = &S.pw[4 * p];
                 // weight / access volume of this contribution (fields.pyx:1472-1488)
-                S.c0[e] = corner_weight(l, wq[0], wq[1], wq[2]) / vol_bin;
                 const int v = big_corner(pr, l, nx, ny);
+                S.c0[e] = corner_weight(l, wq[0], wq[1], wq[2]) / (avol ? avol[v] : vol_bin);
                 uint32_t h = ((uint32_t)v * 2654435761u) >> BIG_HASH_SHIFT;
                 int probes = 0;
                 for (; probes < BIG_HASH; ++probes) {
@@ -383,7 +384,7 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
                     double acc[Bp1];
 #pragma unroll
                     for (int m = 0; m < Bp1; ++m) acc[m] = S.uacc[q * Bp1 + m];
-                    voxel_dE_terms<NB>(R, rho[a], acc, S2, SX, NS);
+                    voxel_dE_terms<NB>(R, vv[a], rho[a], acc, S2, SX, NS);
 #pragma unroll
                     for (int m = 0; m < Bp1; ++m) S.uacc[q * Bp1 + m] = rho[a][m] + acc[m];
                 }

@@ -24,6 +24,7 @@ struct GridGeom {
     int64_t nx, ny, nz;
     double width[3], dxyz[3], half_width[3], half_step[3];
     double vol_bin;
+    const double *access_vol = nullptr;   // per-voxel accessible volumes, or nullptr: vol_bin
 };
 
 // Bead coordinate c of bead b is p[c * sc + b * sb].
@@ -256,6 +257,7 @@ k_dens_gather(GridGeom G, int Bp1, const int32_t *off, const int32_t *cnt, const
     }
     if (t >= nreal) return;
     voxel_cells(G, v, cell, msk);
+    const double vvol = G.access_vol ? G.access_vol[v] : G.vol_bin;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         pos[k] = msk[k] ? off[cell[k]] : 0;
@@ -293,7 +295,7 @@ k_dens_gather(GridGeom G, int Bp1, const int32_t *off, const int32_t *cnt, const
 #pragma unroll
         for (int l = 0; l < 8; ++l) {
             if (!(bm >> l & 1)) continue;
-            const double d = corner_weight(l, wx, wy, wz) / G.vol_bin;
+            const double d = corner_weight(l, wx, wy, wz) / vvol;
             acc[0] += d;
 #pragma unroll
             for (int m = 1; m <= CHROMO_MAX_BINDERS; ++m)

@@ -790,6 +790,15 @@ __device__ __forceinline__ double bead_binding_dE(const chromo_replica *R, const
     return dE;
 }
 
+// Accessible volume of voxel v: the divisor of a bead's weights in it and the
+// volume of its chi term (fields.pyx:1476, 1826); vol_bin unless the field has
+// per-voxel volumes (spherical confinement without assume_fully_accessible).
+__device__ __forceinline__ double voxel_vol(const chromo_replica *R, int64_t v)
+{
+    const double *av = R->access_vol;
+    return av ? av[v] : R->vol_bin;
+}
+
 // ---------------------------------------------------------------------------
 // Field dE terms (chromo/fields.pyx:1675-1842)
 // ---------------------------------------------------------------------------
@@ -798,7 +807,7 @@ __device__ __forceinline__ double bead_binding_dE(const chromo_replica *R, const
 // nonspecific_interact_dE (fields.pyx:1792-1842), added to a lane's running sums.
 // r: current densities, a: delta densities (B+1 species each).
 template <int NB>
-__device__ __forceinline__ void voxel_dE_terms(const chromo_replica *R, const double *r, const double *a,
+__device__ __forceinline__ void voxel_dE_terms(const chromo_replica *R, int64_t v, const double *r, const double *a,
                                                double *S2, double *SX, double &NS)
 {
     for (int j = 0; j < NB; ++j) {
@@ -812,7 +821,7 @@ __device__ __forceinline__ void voxel_dE_terms(const chromo_replica *R, const do
             SX[j * NB + kk] += dx;
         }
     }
-    const double V = R->bead_vol, vf = R->vf_limit, chi = R->chi, vol_bin = R->vol_bin;
+    const double V = R->bead_vol, vf = R->vf_limit, chi = R->chi, vol_bin = voxel_vol(R, v);
     double vf0 = r[0] * V;
     double vf1 = vf0 + (a[0] * V);
     // over vf_limit before and after: the reference's running sum adds +1e99 vf1

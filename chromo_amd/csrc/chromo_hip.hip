@@ -220,6 +220,7 @@ GridGeom grid_of(const chromo_replica &r)
         G.half_step[j] = r.half_step[j];
     }
     G.vol_bin = r.vol_bin;
+    G.access_vol = r.access_vol;
     return G;
 }
 
@@ -239,10 +240,19 @@ __device__ __forceinline__ void field_tape_body(const chromo_replica *R, const i
     else A = make_arena((char *)R->scratch, global_layout(R->bead_cap, R->seg_cap, Bp1));
     const int cap = A.cap;
     const double *Pp = R->pos;
+    // get_confinement_dE (fields.pyx:121-202): beads (spherical) or coordinates
+    // (cubical, trial only) outside the confinement
+    int out_t = 0, out_c = 0;
     for (int64_t i = lane; i < n; i += WAVE) {
         int64_t b = inds[i];
         double o[3] = {Pp[b], Pp[N + b], Pp[2 * N + b]};
         double w[3] = {r_trial[3 * i], r_trial[3 * i + 1], r_trial[3 * i + 2]};
+        if (R->confine_type == 1) {
+            out_t += sqrt(dot3(w, w)) > R->confine_length;
+            out_c += sqrt(dot3(o, o)) > R->confine_length;
+        } else if (R->confine_type == 2) {
+            for (int j = 0; j < 3; ++j) out_t += fabs(w[j]) > R->confine_length / 2;
+        }
         for (int m = 0; m < B; ++m) {
             A.stold[m * cap + i] = R->states[(int64_t)m * N + b];
             A.stnew[m * cap + i] = st_trial[i * B + m];
@@ -254,8 +264,9 @@ __device__ __forceinline__ void field_tape_body(const chromo_replica *R, const i
             voxel_lower(R, pos, ix, wt);
             const int p = (int)(2 * i + k);
             for (int l = 0; l < 8; ++l) {
-                A.evox[p * 8 + l] = (int32_t)corner_voxel(R, ix, l);
-                A.c0[p * 8 + l] = corner_weight(l, wt[0], wt[1], wt[2]) / R->vol_bin;
+                const int64_t cv = corner_voxel(R, ix, l);
+                A.evox[p * 8 + l] = (int32_t)cv;
+                A.c0[p * 8 + l] = corner_weight(l, wt[0], wt[1], wt[2]) / voxel_vol(R, cv);
             }
         }
     }
@@ -316,7 +327,7 @@ __device__ __forceinline__ void field_tape_body(const chromo_replica *R, const i
     double S2[CHROMO_MAX_BINDERS], SX[CHROMO_MAX_BINDERS * CHROMO_MAX_BINDERS], NS = 0;
     for (int j = 0; j < CHROMO_MAX_BINDERS; ++j) S2[j] = 0;
     for (int j = 0; j < CHROMO_MAX_BINDERS * CHROMO_MAX_BINDERS; ++j) SX[j] = 0;
-    const double vol_bin = R->vol_bin, V = R->bead_vol, vf = R->vf_limit, chi = R->chi;
+    const double V = R->bead_vol, vf = R->vf_limit, chi = R->chi;
     for (int q = lane; q < U; q += WAVE) {
         const int s0 = A.uoff[q], s1 = A.uoff[q + 1];
         for (int a = s0 + 1; a < s1; ++a) {
@@ -364,10 +375,11 @@ __device__ __forceinline__ void field_tape_body(const chromo_replica *R, const i
         }
         double vf0 = r[0] * V;
         double vf1 = vf0 + (acc[0] * V);
+        const double av = voxel_vol(R, v);
         if (vf1 > vf) NS += E_HUGE_FIELD * vf1;
-        else NS += chi * (vol_bin / V) * (vf1 * vf1);
+        else NS += chi * (av / V) * (vf1 * vf1);
         if (vf0 > vf) NS -= E_HUGE_FIELD * vf0;
-        else NS -= chi * (vol_bin / V) * (vf0 * vf0);
+        else NS -= chi * (av / V) * (vf0 * vf0);
     }
     for (int j = 0; j < B; ++j) S2[j] = wave_sum(S2[j]);
     for (int j = 0; j < B * B; ++j) SX[j] = wave_sum(SX[j]);
@@ -401,8 +413,15 @@ __device__ __forceinline__ void field_tape_body(const chromo_replica *R, const i
         A.hkey[s] = HKEY_EMPTY;
         A.hmin[s] = HMIN_EMPTY;
     }
+    out_t = bcast0(wave_sum_int(out_t));
+    out_c = bcast0(wave_sum_int(out_c));
+    double dE = 0.0;
+    if (!state_change) {
+        dE += (double)out_t * E_HUGE_FIELD;
+        dE -= (double)out_c * E_HUGE_FIELD;
+    }
     if (lane == 0) {
-        *dE_out = 0.0 + dEb;
+        *dE_out = dE + dEb;
         *nbins = U;
     }
 }
@@ -490,6 +509,71 @@ __global__ void k_cexp_tape(const int32_t *draws, const int64_t *windows, int64_
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = cexp_trunc(draws[i], windows[i]);
 }
+
+// ---- checkpoint / restore -----------------------------------------------------
+//
+// File: "CHROMOCK" magic, u32 version, i32 replica count; per replica a header
+// (N, B, n_ctrl, n_bins, rng_mode) and its state: positions (9 N doubles, as
+// stored), binder states (B N bytes), voxel densities (n_bins (B+1) doubles),
+// RNG block (CHROMO_RNG_WORDS words), controllers (n_ctrl records), counters.
+// Marks, bond parameters and the field's parameters are inputs, not state: the
+// table that restores must describe the same chains and fields.
+
+namespace {
+
+constexpr char CK_MAGIC[8] = {'C', 'H', 'R', 'O', 'M', 'O', 'C', 'K'};
+constexpr uint32_t CK_VERSION = 1;
+
+struct CkHeader {
+    int64_t N, n_bins;
+    int32_t B, n_ctrl, rng_mode, pad;
+};
+
+struct CkArrays {
+    void *dev;
+    size_t bytes;
+};
+
+CkHeader ck_header(const chromo_replica &r)
+{
+    CkHeader h{};
+    h.N = r.N;
+    h.n_bins = r.nx * r.ny * r.nz;
+    h.B = r.B;
+    h.n_ctrl = r.n_ctrl;
+    h.rng_mode = r.rng_mode;
+    return h;
+}
+
+void ck_arrays(const chromo_replica &r, CkArrays a[6])
+{
+    a[0] = {r.pos, sizeof(double) * 9 * (size_t)r.N};
+    a[1] = {r.states, (size_t)r.B * (size_t)r.N};
+    a[2] = {r.density, sizeof(double) * (size_t)(r.nx * r.ny * r.nz) * (size_t)(r.B + 1)};
+    a[3] = {r.rng, sizeof(uint32_t) * CHROMO_RNG_WORDS};
+    a[4] = {r.ctrl, sizeof(chromo_ctrl) * (size_t)r.n_ctrl};
+    a[5] = {r.counters, sizeof(int64_t) * CHROMO_CNT_WORDS};
+}
+
+struct File {
+    FILE *f = nullptr;
+    ~File()
+    {
+        if (f) fclose(f);
+    }
+};
+
+int read_table(const chromo_replica *d_reps, int32_t n_reps, hipStream_t st, std::vector<chromo_replica> &h)
+{
+    int rc = check_reps(d_reps, n_reps);
+    if (rc) return rc;
+    h.resize(n_reps);
+    HIPC(hipMemcpyAsync(h.data(), d_reps, sizeof(chromo_replica) * n_reps, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    return 0;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -708,6 +792,86 @@ int chromo_field_dE_tape(const chromo_replica *d_reps, int32_t rep, const int64_
                        d_r_trial, d_states_trial, state_change, d_dE, d_bins, d_delta, d_n_bins,
                        g_lds_bp1);
     HIPC(launch_err());
+    return 0;
+}
+
+int chromo_checkpoint(const chromo_replica *d_reps, int32_t n_reps, const char *path, void *stream)
+{
+    if (!path) return set_err(-1, "path is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<chromo_replica> h;
+    int rc = read_table(d_reps, n_reps, st, h);
+    if (rc) return rc;
+    File out;
+    out.f = fopen(path, "wb");
+    if (!out.f) return set_err(-1, std::string("cannot open ") + path + " for writing");
+    bool ok = fwrite(CK_MAGIC, 1, 8, out.f) == 8 && fwrite(&CK_VERSION, 4, 1, out.f) == 1 &&
+              fwrite(&n_reps, 4, 1, out.f) == 1;
+    std::vector<char> buf;
+    for (int32_t i = 0; i < n_reps && ok; ++i) {
+        const CkHeader hd = ck_header(h[i]);
+        ok = fwrite(&hd, sizeof(hd), 1, out.f) == 1;
+        CkArrays a[6];
+        ck_arrays(h[i], a);
+        for (int k = 0; k < 6 && ok; ++k) {
+            buf.resize(a[k].bytes);
+            if (a[k].bytes) {
+                HIPC(hipMemcpyAsync(buf.data(), a[k].dev, a[k].bytes, hipMemcpyDeviceToHost, st));
+                HIPC(hipStreamSynchronize(st));
+                ok = fwrite(buf.data(), 1, a[k].bytes, out.f) == a[k].bytes;
+            }
+        }
+    }
+    if (!ok) return set_err(-1, std::string("write to ") + path + " failed");
+    if (fflush(out.f) != 0) return set_err(-1, std::string("write to ") + path + " failed");
+    return 0;
+}
+
+int chromo_restore(const chromo_replica *d_reps, int32_t n_reps, const char *path, void *stream)
+{
+    if (!path) return set_err(-1, "path is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<chromo_replica> h;
+    int rc = read_table(d_reps, n_reps, st, h);
+    if (rc) return rc;
+    File in;
+    in.f = fopen(path, "rb");
+    if (!in.f) return set_err(-1, std::string("cannot open ") + path);
+    char magic[8];
+    uint32_t ver = 0;
+    int32_t n = 0;
+    if (fread(magic, 1, 8, in.f) != 8 || memcmp(magic, CK_MAGIC, 8) != 0 || fread(&ver, 4, 1, in.f) != 1 ||
+        fread(&n, 4, 1, in.f) != 1)
+        return set_err(-1, std::string(path) + " is not a chromo checkpoint");
+    if (ver != CK_VERSION) return set_err(-1, "checkpoint version " + std::to_string(ver) + " is not supported");
+    if (n != n_reps)
+        return set_err(-1, "checkpoint holds " + std::to_string(n) + " replicas, the table " + std::to_string(n_reps));
+    // read and check everything before writing any device memory
+    std::vector<std::vector<char>> data((size_t)n_reps * 6);
+    for (int32_t i = 0; i < n_reps; ++i) {
+        CkHeader hd{}, want = ck_header(h[i]);
+        if (fread(&hd, sizeof(hd), 1, in.f) != 1) return set_err(-1, std::string(path) + " is truncated");
+        if (hd.N != want.N || hd.B != want.B || hd.n_ctrl != want.n_ctrl || hd.n_bins != want.n_bins ||
+            hd.rng_mode != want.rng_mode)
+            return set_err(-1, "replica " + std::to_string(i) +
+                                   ": checkpoint chain, grid, controllers or RNG mode differ from the table");
+        CkArrays a[6];
+        ck_arrays(h[i], a);
+        for (int k = 0; k < 6; ++k) {
+            auto &v = data[(size_t)i * 6 + k];
+            v.resize(a[k].bytes);
+            if (a[k].bytes && fread(v.data(), 1, a[k].bytes, in.f) != a[k].bytes)
+                return set_err(-1, std::string(path) + " is truncated");
+        }
+    }
+    for (int32_t i = 0; i < n_reps; ++i) {
+        CkArrays a[6];
+        ck_arrays(h[i], a);
+        for (int k = 0; k < 6; ++k)
+            if (a[k].bytes)
+                HIPC(hipMemcpyAsync(a[k].dev, data[(size_t)i * 6 + k].data(), a[k].bytes, hipMemcpyHostToDevice, st));
+    }
+    HIPC(hipStreamSynchronize(st));
     return 0;
 }
 

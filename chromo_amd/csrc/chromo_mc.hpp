@@ -351,9 +351,10 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, double u, Pro
                 A.spk[p] = sp;
 #pragma unroll
                 for (int l = 0; l < 8; ++l) {
-                    A.evox[p * 8 + l] = (int32_t)corner_voxel(R, ix, l);
+                    const int64_t cv = corner_voxel(R, ix, l);
+                    A.evox[p * 8 + l] = (int32_t)cv;
                     // weight / access volume of this contribution (fields.pyx:1472-1488)
-                    A.c0[p * 8 + l] = corner_weight(l, wt[0], wt[1], wt[2]) / vol_bin;
+                    A.c0[p * 8 + l] = corner_weight(l, wt[0], wt[1], wt[2]) / (R->access_vol ? R->access_vol[cv] : vol_bin);
                 }
             }
         }
@@ -517,7 +518,7 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, double u, Pro
                 A.udrho[m * A.ecap + q] = acc[m];
                 r[m] = A.urho[m * A.ecap + q];
             }
-            voxel_dE_terms<NB>(R, r, acc, S2, SX, NS);
+            voxel_dE_terms<NB>(R, A.uvox[q], r, acc, S2, SX, NS);
         }
         const double share = wave_sum(lane_field_share<NB>(R, S2, SX, NS));
         // count_doubly_bound (fields.pyx:1877-1937) and confinement (:121-202)

@@ -391,6 +391,7 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
     if (check_field) {
         const int nx = (int)R->nx, ny = (int)R->ny, nz = (int)R->nz;
         const double vol_bin = R->vol_bin;
+        const double *avol = R->access_vol;
         // positions 2i (current) and 2i + 1 (trial; the current one for a state change)
         if (act) {
 #pragma unroll
@@ -426,11 +427,11 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
                 const int l = e & 7;
                 // weight / access volume of this contribution (fields.pyx:1472-1488)
                 const double *wq = &S.pw[4 * (e >> 3)];
-                S.c0[e] = corner_weight(l, wq[0], wq[1], wq[2]) / vol_bin;
                 const int x = (l & 1) ? (pr.x >> 16) : (pr.x & 0xffff);
                 const int y = (l & 2) ? (pr.y >> 16) : (pr.y & 0xffff);
                 const int z = (l & 4) ? (pr.z >> 16) : (pr.z & 0xffff);
                 const int v = x + nx * (y + ny * z);
+                S.c0[e] = corner_weight(l, wq[0], wq[1], wq[2]) / (avol ? avol[v] : vol_bin);
                 uint32_t h = ((uint32_t)v * 2654435761u) >> SEG_HASH_SHIFT;
                 // fewer distinct keys than slots before each insertion: the probe ends
                 for (int probes = 0; probes < SEG_HASH; ++probes) {
@@ -520,8 +521,10 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
             uint64_t hits[MW];
 #pragma unroll
             for (int w = 0; w < MW; ++w) hits[w] = 0;
+            int vown = 0;
             if (own) {
                 const int v = S.uvox[q];
+                vown = v;
 #pragma unroll
                 for (int m = 0; m < Bp1; ++m) rho[m] = rho_g[(int64_t)v * Bp1 + m];
                 const int2 c = S.ucrd[q];
@@ -568,7 +571,7 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
                 }
             }
             if (own) {
-                voxel_dE_terms<NB>(R, rho, acc, S2, SX, NS);
+                voxel_dE_terms<NB>(R, vown, rho, acc, S2, SX, NS);
 #pragma unroll
                 for (int m = 0; m < Bp1; ++m) K.nr[j][m] = rho[m] + acc[m];
             }

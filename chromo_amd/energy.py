@@ -58,6 +58,9 @@ class EnergyModel:
     def __init__(self, chain: ChainSpec, field: FieldSpec, binders, device=None):
         if not field.field_on:
             raise ValueError("the energy model needs a UniformDensityField")
+        if getattr(field, "access_vol", None) is not None:
+            raise NotImplementedError("the energy adjoint takes every voxel at its full volume "
+                                      "(no per-voxel accessible volumes)")
         self.lib = L.lib()
         dev = device or torch.device("cuda", torch.cuda.current_device())
         self.device = dev

@@ -66,6 +66,7 @@ class FieldSpec:
     cross: Optional[np.ndarray] = None   # (B, B)
     confine_type: int = 0
     confine_length: float = 0.0
+    access_vol: Optional[np.ndarray] = None   # (n_bins,) accessible volumes, None: vol_bin
 
     @property
     def n_bins(self) -> int:
@@ -241,6 +242,19 @@ class DeviceReplicas:
         self.marks = torch.from_numpy(mk).to(dev)
         self.bonds = torch.from_numpy(bo).to(dev)
         self.density = torch.zeros(int(self.o_den[-1]), dtype=torch.float64, device=dev)
+        # accessible volumes of the fields that have them (spherical confinement
+        # without assume_fully_accessible), one flat buffer
+        av_list = []
+        for f in fields:
+            a = getattr(f, "access_vol", None) if f.field_on else None
+            if a is not None:
+                a = np.asarray(a, dtype=np.float64).reshape(-1)
+                if a.size != f.n_bins:
+                    raise ValueError("access_vol must hold one volume per voxel")
+            av_list.append(a)
+        self.o_av = offsets([0 if a is None else a.size for a in av_list])
+        self.access_vol = (torch.from_numpy(np.concatenate([a for a in av_list if a is not None])).to(dev)
+                           if any(a is not None for a in av_list) else None)
         self.rng = torch.zeros(n * L.RNG_WORDS, dtype=torch.int32, device=dev)
         cs = ctypes.sizeof(L.ChromoCtrl)
         ctrl_host = (L.ChromoCtrl * int(self.o_ctrl[-1]))()
@@ -298,6 +312,8 @@ class DeviceReplicas:
                         R.cross[a * B + b2] = float(cross[a, b2])
                 R.confine_type = int(f.confine_type)
                 R.confine_length = float(f.confine_length)
+                if av_list[i] is not None:
+                    R.access_vol = base(self.access_vol) + 8 * int(self.o_av[i])
             else:
                 R.nx = R.ny = R.nz = 1
             for a in range(B):
@@ -400,6 +416,17 @@ class DeviceReplicas:
         if bad.size:
             code = int(cnt[bad[0]])
             raise L.ChromoError(f"replica {int(bad[0])}: {L.ERROR_TEXT.get(code, code)}")
+
+    # -- checkpoint ----------------------------------------------------------
+    def checkpoint(self, path) -> None:
+        """Every replica's state (chains, states, densities, RNG, controllers,
+        counters) to ``path``; ``restore`` continues it bit for bit."""
+        L.check(self.lib.chromo_checkpoint(self.tab_ptr, self.n, str(path).encode(), _current_stream()),
+                "chromo_checkpoint")
+
+    def restore(self, path) -> None:
+        L.check(self.lib.chromo_restore(self.tab_ptr, self.n, str(path).encode(), _current_stream()),
+                "chromo_restore")
 
     # -- readback ------------------------------------------------------------
     def positions(self, i: int):

@@ -151,6 +151,8 @@ typedef struct {
     int32_t confine_type;    /* 0 none, 1 spherical, 2 cubical */
     double confine_length;
     double F[CHROMO_MAX_BINDERS][CHROMO_MAX_MARK][CHROMO_MAX_SITES + 1];
+    const double *access_vol;   /* n_bins accessible volumes (get_accessible_volumes,
+                                   chromo/fields.pyx:714-770), or NULL: every voxel vol_bin */
 } chromo_replica;
 
 /* ---- library ---------------------------------------------------------- */
@@ -213,6 +215,20 @@ int chromo_mc_step(const chromo_replica *d_reps, int32_t rep, int32_t ctrl, void
 /* SimpleControl.update_amplitudes (chromo/mc/mc_controller.py:126-213). */
 int chromo_update_amplitudes(const chromo_replica *d_reps, int32_t rep, int32_t ctrl,
                              void *stream);
+
+/* ---- checkpoint / restore ----------------------------------------------- */
+
+/* Writes every replica's state to `path`: positions, binder states, voxel
+ * densities, the RNG block (glibc / MT19937 / Philox position), controllers and
+ * counters.  Continuing from chromo_restore gives the same chains bit for bit as
+ * running on.  The reference resumes from its CSV snapshots with fresh streams
+ * (continue_polymer_in_field_simulation, chromo/mc/__init__.py:167-227); this
+ * entry has no single counterpart and adds the exact continuation. */
+int chromo_checkpoint(const chromo_replica *d_reps, int32_t n_reps, const char *path, void *stream);
+/* Reads a chromo_checkpoint file into a table describing the same chains,
+ * fields, controllers and RNG modes (checked; nothing is written on a
+ * mismatch). */
+int chromo_restore(const chromo_replica *d_reps, int32_t n_reps, const char *path, void *stream);
 
 /* ---- tape entry points (parity testing) -------------------------------- */
 

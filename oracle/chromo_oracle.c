@@ -363,6 +363,7 @@ struct or_sim {
     double confine_length;
     int64_t *wrap;                 /* inds_xyz_to_super, (nx+1)(ny+1)(nz+1) */
     double *density, *density_trial;
+    double *access_vols;           /* access_vols[bin] (fields.pyx:714-770) */
     int64_t *affected;
     int faithful;
     /* move scratch */
@@ -449,6 +450,9 @@ or_sim *or_new(const or_spec *sp, int32_t rng_mode, uint32_t seed, uint32_t stre
     s->dxyz[2] = sp->z_width / (double)s->nz;
     s->n_bins = s->nx * s->ny * s->nz;
     s->vol_bin = sp->x_width * sp->y_width * sp->z_width / (double)s->n_bins;
+    s->access_vols = (double *)malloc((size_t)s->n_bins * sizeof(double));
+    for (int64_t i = 0; i < s->n_bins; ++i)
+        s->access_vols[i] = sp->access_vols ? sp->access_vols[i] : s->vol_bin;
     s->width[0] = sp->x_width;
     s->width[1] = sp->y_width;
     s->width[2] = sp->z_width;
@@ -517,6 +521,7 @@ void or_free(or_sim *s)
     free(s->eta);
     free(s->wrap);
     free(s->density);
+    free(s->access_vols);
     free(s->density_trial);
     free(s->affected);
     free(s->inds);
@@ -1125,7 +1130,7 @@ static int64_t change_in_density(or_sim *s, const double *r_trial, const int64_t
         }
         for (int k = 0; k < 2; ++k)
             for (int l = 0; l < 8; ++l) {
-                dt[k][0][l] = wt[k][l] / s->vol_bin;
+                dt[k][0][l] = wt[k][l] / s->access_vols[idx[k][l]];
                 for (int64_t m = 1; m < Bp1; ++m) {
                     int64_t st = (k == 0 || state_change == 0) ? s->states[b * B + m - 1]
                                                                : st_trial[b * B + m - 1];
@@ -1218,7 +1223,7 @@ static double dE_binders_and_beads(const or_sim *s, const int64_t *st_trial, con
     double ns = 0;
     double V = s->bead_vol;
     for (int64_t i = 0; i < nb; ++i) {
-        double av = s->vol_bin;
+        double av = s->access_vols[bins[i]];
         double vf0 = s->density[bins[i] * Bp1] * V;
         double vf1 = vf0 + (s->density_trial[bins[i] * Bp1] * V);
         if (vf1 > (double)s->vf_limit) ns += E_HUGE_FIELD * vf1;
@@ -1281,7 +1286,7 @@ void or_density_init(or_sim *s)
         double wt[8];
         bead_voxels(s, s->r + 3 * i, idx, wt);
         for (int l = 0; l < 8; ++l) {
-            double d = wt[l] / s->vol_bin;
+            double d = wt[l] / s->access_vols[idx[l]];
             s->density[idx[l] * Bp1] += d;
             for (int64_t m = 1; m < Bp1; ++m)
                 s->density[idx[l] * Bp1 + m] += d * (double)s->states[i * B + m - 1];
@@ -1581,7 +1586,7 @@ double or_field_E(or_sim *s)
         double wt[8];
         bead_voxels(s, s->r + 3 * i, idx, wt);
         for (int l = 0; l < 8; ++l) {
-            double d = wt[l] / s->vol_bin;
+            double d = wt[l] / s->access_vols[idx[l]];
             s->density[idx[l] * Bp1] += d;
             for (int64_t m = 1; m < Bp1; ++m)
                 s->density[idx[l] * Bp1 + m] += d * (double)s->states[i * B + m - 1];
@@ -1601,7 +1606,7 @@ double or_field_E(or_sim *s)
     for (int64_t i = 0; i < s->n_bins; ++i) {
         double vf = s->density[i * Bp1] * V;
         if (py_round2(vf) > (double)s->vf_limit) ns += E_HUGE_FIELD * vf;
-        else ns += s->chi * (s->vol_bin / V) * vf * (1 - vf);
+        else ns += s->chi * (s->access_vols[i] / V) * vf * (1 - vf);
     }
     E += ns;
     return E;

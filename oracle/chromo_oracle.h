@@ -44,6 +44,7 @@ typedef struct {
     double pref[OR_MAXB], intra[OR_MAXB], cross[OR_MAXB * OR_MAXB];
     int32_t confine_type;               /* 0 none, 1 spherical, 2 cubical */
     double confine_length;
+    const double *access_vols;          /* n_bins accessible volumes (copied), or NULL: vol_bin */
 } or_spec;
 
 typedef struct {

@@ -40,6 +40,7 @@ class OrSpec(ctypes.Structure):
         ("bead_vol", ctypes.c_double), ("pref", ctypes.c_double * OR_MAXB),
         ("intra", ctypes.c_double * OR_MAXB), ("cross", ctypes.c_double * (OR_MAXB * OR_MAXB)),
         ("confine_type", ctypes.c_int32), ("confine_length", ctypes.c_double),
+        ("access_vols", _dp),
     ]
 
 
@@ -197,6 +198,11 @@ class OracleSim:
                 sp.cross[a * B + b] = float(cross[a, b])
         sp.confine_type = int(field.confine_type)
         sp.confine_length = float(field.confine_length)
+        av = getattr(field, "access_vol", None)
+        if av is not None and field.field_on:
+            av = _d(av).reshape(-1)
+            keep(av)
+            sp.access_vols = _dptr(av)
         self.N, self.B = N, B
         self.h = h
         self.p = h.or_new(ctypes.byref(sp), int(rng_mode), int(seed) & 0xffffffff,

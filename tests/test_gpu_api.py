@@ -71,9 +71,35 @@ def test_polymer_in_field_snapshots_roundtrip(tmp_path):
     bead_b, move_b = mc.get_amplitude_bounds([p])
     mc.polymer_in_field([p], binders, udf, 3, 2, bead_b, move_b, random_seed=5,
                         output_dir=str(tmp_path))
-    q = Chromatin.from_file(str(tmp_path / "chr-1.csv"))
+    run = tmp_path / "sim_1"
+    q = Chromatin.from_file(str(run / "chr-1.csv"))
     assert np.allclose(q.r, p.r, rtol=0, atol=1e-12) and np.array_equal(q.states, p.states)
     assert np.array_equal(q.chemical_mods, p.chemical_mods)
+    # acceptance-tracker files per controller and save point, the field description
+    assert (run / "acceptance_trackers" / "crank_shaft_snap_1.csv").exists()
+    assert (run / "field").exists()
+
+
+def test_continue_polymer_in_field_simulation(tmp_path):
+    """Resume from the latest snapshot of the latest run: a new sim_2 folder that
+    starts from sim_1's last configuration (chromo/mc/__init__.py:167-227)."""
+    say("continue simulation")
+    from chromo_amd import mc
+    from chromo_amd.fields import UniformDensityField
+    from chromo_amd.polymers import Chromatin
+    p, binders, udf = _script_system(N=200, n_vox=6, seed=3)
+    bead_b, move_b = mc.get_amplitude_bounds([p])
+    mc.polymer_in_field([p], binders, udf, 2, 2, bead_b, move_b, random_seed=5, output_dir=str(tmp_path))
+    last = Chromatin.from_file(str(tmp_path / "sim_1" / "chr-1.csv"))
+    field2 = UniformDensityField([last], binders, udf.x_width, udf.nx, udf.y_width, udf.ny,
+                                 udf.z_width, udf.nz)
+    polys = mc.continue_polymer_in_field_simulation(Chromatin, binders, field2, str(tmp_path), 2, 1,
+                                                    random_seed=6)
+    assert (tmp_path / "sim_2" / "chr-0.csv").exists()
+    assert polys[0].name == "chr" and polys[0].num_beads == 200
+    # the field read back from the run's description lists the same polymer and binders
+    f3 = UniformDensityField.from_file(str(tmp_path / "sim_2" / "field"), polys, binders)
+    assert (f3.nx, f3.x_width, f3.chi) == (udf.nx, udf.x_width, udf.chi)
 
 
 def test_plain_launch_matches_persistent():

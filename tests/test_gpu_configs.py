@@ -243,3 +243,39 @@ def test_capped_exponential_truncation_matches_glibc(oracle):
     bad = np.nonzero(dev != host)[0]
     say(f"  {len(d)} draws, {bad.size} differ")
     assert bad.size == 0, (d[bad[:5]], w[bad[:5]], dev[bad[:5]], host[bad[:5]])
+
+
+def test_accessible_volumes_spherical_confinement(oracle):
+    """Spherical confinement with assume_fully_accessible = 0: the voxels the
+    sphere's surface splits get the fraction of their 20^3 subgrid inside
+    (chromo/fields.pyx:714-951), which divides every bead weight in them and sets
+    their chi volume.  The initial densities bit for bit, the tape (delta
+    densities bit for bit) and a run of every move type against the oracle."""
+    from chromo_amd.binders import hp1, make_binder_collection
+    from chromo_amd.fields import UniformDensityField
+    say("accessible volumes")
+    s = W.make_system(1200, 2, 24, seed=65)
+    s.chain.r -= s.chain.r.mean(axis=0)
+    R = 300.0
+    f = s.field
+    udf = UniformDensityField([], make_binder_collection([hp1]), f.x_width, f.nx, f.y_width, f.ny,
+                              f.z_width, f.nz, confine_type="Spherical", confine_length=R,
+                              assume_fully_accessible=0)
+    f.access_vol = udf.access_vol_array()
+    f.confine_type, f.confine_length = 1, R
+    assert f.access_vol is not None and np.sum(f.access_vol < udf.vol_bin) > 100
+    o = oracle.OracleSim(s.chain, s.field)
+    dr = _dev([s])
+    assert np.array_equal(dr.density_of(0), o.density())
+    rs = np.random.RandomState(3)
+    for n in (1, 9, 40):
+        ind0 = rs.randint(0, 1200 - n)
+        inds = np.arange(ind0, ind0 + n)
+        rt = s.chain.r[inds] + rs.normal(scale=15.0, size=(n, 3))
+        st = s.chain.states[inds]
+        dEo, bins_o, delta_o = o.field_dE_tape(inds, rt, st, 0)
+        dEg, bins_g, delta_g = dr.field_dE_tape(0, inds, rt, st.astype(np.int8), False)
+        assert sorted(bins_g) == sorted(bins_o)
+        assert np.array_equal(delta_g, delta_o[bins_g])
+        assert _dE_close(np.array([dEg]), np.array([dEo]))[0]
+    _run_pair(oracle, s, 8, waves=2, rel=1e-9)

@@ -123,3 +123,25 @@ def test_moves_are_device_markers():
     assert moves.crank_shaft.__name__ == "crank_shaft"
     with pytest.raises(TypeError):
         moves.slide(None, 1.0, 1)
+
+
+def test_mu_schedules():
+    """The annealing factors at the start, middle and end of a run, against the
+    closed forms of chromo/util/mu_schedules.py."""
+    import math
+    from chromo_amd.util import mu_schedules as M
+    N = 120
+    assert M.linear_1(0, N) == -1 and M.linear_1(N, N) == 1
+    assert M.linear_2(30, N) == -1 and M.linear_2(31, N) == 2 * (1 / 90) - 1 and M.linear_2(N, N) == 1
+    assert M.linear_3(20, N) == -1 and M.linear_4(15, N) == -1 and M.linear_4(N, N) == 1
+    assert math.isclose(M.tanh_1(0, N), math.tanh(-3)) and math.isclose(M.tanh_2(N, N), math.tanh(2))
+    assert M.tanh_3(20, N) == math.tanh(-1) and math.isclose(M.tanh_4(N, N), math.tanh(2))
+    assert math.isclose(M.log_1(0, N), -1) and math.isclose(M.log_2(N, N), math.log(4.4) - 0.5)
+    assert math.isclose(M.log_3(0, N), math.log(0.5) - 0.098)
+    assert M.sawtooth_1(N - 1, N) == 1 and math.isclose(M.sawtooth_1(40, N), math.log(40 / 12 + 0.5) - 0.34)
+    assert M.linear_2_for_negative_cp(0, N) == 4.0 and M.linear_2_for_negative_cp(N, N) == 1.0
+    assert M.linear_step_for_negative_cp(24, N) == 4.0 and M.linear_step_for_negative_cp(96, N) == 1.0
+    assert math.isclose(M.linear_step_for_negative_cp(60, N), 4 - 3 * 36 / 72)
+    assert M.linear_step_for_negative_cp_mild(0, N) == 2.0 and M.linear_step_for_negative_cp_mild(N, N) == 1.0
+    s = M.Schedule(M.tanh_3)
+    assert s.name == "tanh_3" and s.function(0, N) == math.tanh(-1)

@@ -265,3 +265,96 @@ def test_cexp_trunc_equals_python_next_to_crossings(oracle):
     assert np.array_equal(got, np.array(exp))
     # the draws do straddle each crossing: both truncations occur around every x_j
     assert len(np.unique(got)) > 300
+
+
+# ---------------------------------------------------- accessible volumes ---
+
+def _sphere_field(R=290.0, n=24, voxel=30.0):
+    from chromo_amd.binders import hp1, make_binder_collection
+    from chromo_amd.fields import UniformDensityField
+    W_ = n * voxel
+    return UniformDensityField([], make_binder_collection([hp1]), W_, n, W_, n, W_, n,
+                               confine_type="Spherical", confine_length=R, assume_fully_accessible=0)
+
+
+def test_accessible_volumes_of_a_sphere():
+    """Voxels inside the sphere keep vol_bin, split voxels get the fraction of
+    their 20^3 subgrid inside, and the inside plus split volumes add up to the
+    sphere's volume 4/3 pi R^3 (to the subgrid's resolution); voxels outside keep
+    vol_bin (chromo/fields.pyx:752-770 only changes split voxels)."""
+    R = 290.0
+    f = _sphere_field(R)
+    v = f.access_vol_array()
+    n, dx = f.nx, f.dx
+    i = np.arange(f.n_bins)
+    c = (np.stack([i % n, (i // n) % n, i // (n * n)], 1) - (n - 1) / 2) * dx
+    d = np.linalg.norm(c, axis=1)
+    buf = np.sqrt(2) / 4 * dx
+    inner, outer = d < R - buf, d > R + buf
+    split = ~inner & ~outer
+    assert np.all(v[inner] == f.vol_bin) and np.all(v[outer] == f.vol_bin)
+    assert np.all(v[split] <= f.vol_bin) and np.any(v[split] < f.vol_bin)
+    total = v[inner].sum() + v[split].sum()
+    assert abs(total / (4 / 3 * np.pi * R ** 3) - 1) < 0.02
+    assert f.access_vols[int(np.nonzero(split)[0][0])] == v[split][0]
+    f2 = _sphere_field(R)
+    assert f2.access_vol_array() is not None
+    from chromo_amd.binders import hp1, make_binder_collection
+    from chromo_amd.fields import UniformDensityField
+    flat = UniformDensityField([], make_binder_collection([hp1]), 720, 24, 720, 24, 720, 24,
+                               confine_type="Cubical", confine_length=600, assume_fully_accessible=0)
+    assert flat.access_vol_array() is None          # only spherical confinements have split voxels
+
+
+def _density_bruteforce_av(field, r, states, av):
+    n = np.array([field.nx, field.ny, field.nz])
+    Wd = np.array([field.x_width, field.y_width, field.z_width])
+    dx = Wd / n
+    B = states.shape[1]
+    rho = np.zeros((n.prod(), B + 1))
+    q = (np.mod(r + Wd / 2, Wd) - dx / 2) / dx
+    lo = np.floor(q).astype(int)
+    w = 1 - (q - lo)
+    for l in range(8):
+        o = np.array([l & 1, (l >> 1) & 1, (l >> 2) & 1])
+        idx = np.mod(lo + o, n)
+        wt = np.prod(np.where(o == 0, w, 1 - w), axis=1)
+        v = idx[:, 0] + idx[:, 1] * n[0] + idx[:, 2] * n[0] * n[1]
+        np.add.at(rho[:, 0], v, wt / av[v])
+        for m in range(B):
+            np.add.at(rho[:, m + 1], v, wt / av[v] * states[:, m])
+    return rho
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_field_dE_with_accessible_volumes_bruteforce(oracle, seed):
+    """Per-voxel accessible volumes divide each bead's weights and set the chi
+    term's volume (fields.pyx:1476, 1826): densities and dE against a
+    whole-grid recomputation with random volumes."""
+    s = W.make_system(80, 2, 5, seed=seed, eps=[-3.0, -1.5], cross_eps=0.8, chi=1.3)
+    rs = np.random.RandomState(seed)
+    vb = s.field.x_width ** 3 / 125
+    s.field.access_vol = vb * rs.uniform(0.3, 1.0, size=125)
+    st = rs.randint(0, 3, size=(80, 2))
+    s.chain.states = st
+    sim = oracle.OracleSim(s.chain, s.field)
+    av = s.field.access_vol
+    rho0 = _density_bruteforce_av(s.field, s.chain.r, st, av)
+    assert np.allclose(sim.density(), rho0, rtol=1e-12, atol=1e-20)
+    inds = np.arange(10, 16)
+    rt = s.chain.r[inds] + rs.normal(scale=9.0, size=(6, 3))
+    dE, bins, delta = sim.field_dE_tape(inds, rt, st[inds], 0)
+    r2 = s.chain.r.copy()
+    r2[inds] = rt
+    rho1 = _density_bruteforce_av(s.field, r2, st, av)
+    assert np.allclose(rho0 + delta, rho1, rtol=1e-10, atol=1e-15)
+
+    def energy(rho):
+        E = 0.0
+        for j in range(2):
+            E += s.field.pref[j] * np.sum(rho[:, j + 1] ** 2)
+            for k in range(2):
+                E += s.field.cross[j, k] * np.sum(rho[:, j + 1] * rho[:, k + 1])
+        phi = rho[:, 0] * s.field.bead_vol
+        return E + np.sum(s.field.chi * (av / s.field.bead_vol) * phi ** 2)
+    assert abs(dE - (energy(rho1) - energy(rho0))) < 1e-8 * max(1, abs(energy(rho1)))
