@@ -84,6 +84,9 @@ class ChromoReplica(ctypes.Structure):
         ("confine_length", ctypes.c_double),
         ("F", ctypes.c_double * (MAX_BINDERS * MAX_MARK * (MAX_SITES + 1))),
         ("access_vol", ctypes.c_void_p),
+        ("fast_field", ctypes.c_int32), ("pad_ff", ctypes.c_int32),
+        ("n_sub", ctypes.c_int64 * 3), ("sub_width", ctypes.c_double * 3),
+        ("half_points", ctypes.c_double),
     ]
 
 

@@ -126,6 +126,7 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
     const bool degen = nx == 1 || ny == 1 || nz == 1;
     const double vol_bin = R->vol_bin;
     const double *avol = R->access_vol;
+    const bool thr = R->fast_field == 0;
     const int ct = R->confine_type;
     const double cl = R->confine_length;
     res.overflow = 0;
@@ -328,7 +329,7 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
                           z0 = pr.z & 0xffff, z1 = pr.z >> 16;
                 if (!degen) {
                     const int l = (vx == x1 ? 1 : 0) | (vy == y1 ? 2 : 0) | (vz == z1 ? 4 : 0);
-                    fold_c0<NB>(S.c0[p * 8 + l], pre, pr.w, acc);
+                    fold_c0<NB>(S.c0[p * 8 + l], pre, pr.w, acc, thr);
                 } else {
                     int dx = vx - x0, dy = vy - y0, dz = vz - z0;
                     if (dx < 0) dx += nx;
@@ -337,7 +338,7 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
                     for (int l = 0; l < 8; ++l)
                         if (axis_hit(l & 1, dx, nx) && axis_hit((l >> 1) & 1, dy, ny) &&
                             axis_hit((l >> 2) & 1, dz, nz))
-                            fold_c0<NB>(S.c0[p * 8 + l], pre, pr.w, acc);
+                            fold_c0<NB>(S.c0[p * 8 + l], pre, pr.w, acc, thr);
                 }
             }
             if (own) {

@@ -876,6 +876,26 @@ __device__ __forceinline__ void voxel_lower(const chromo_replica *R, const doubl
                                             double w[3])
 {
     const int64_t nn[3] = {R->nx, R->ny, R->nz};
+    if (R->fast_field) {
+        // the position's sub-bin floor((x + W/2) / sbw) mod (n_points n) and that
+        // sub-bin's lower voxel and weight, from init_fast_field's expressions
+        // (fields.pyx:614-671, 1255-1300)
+        const double half = R->half_points;
+        for (int j = 0; j < 3; ++j) {
+            const double sw = R->sub_width[j];
+            const double qs = (pos[j] + R->half_width[j]) / sw;
+            const double sb = isfinite(qs) ? floor(qs) : 0.0;
+            if (fabs(qs - rint(qs)) < TIE_VOXEL_EPS)
+                atomicAdd((unsigned long long *)&R->counters[CHROMO_CNT_TIE_VOXEL], 1ull);
+            int64_t i = (fabs(sb) < 0x1p62) ? (int64_t)sb % R->n_sub[j] : 0;
+            if (i < 0) i += R->n_sub[j];
+            const double u = (((double)i - half) * sw) / R->dxyz[j];
+            const double ind = floor(u);
+            ix[j] = (ind == -1.0) ? nn[j] - 1 : (int64_t)ind;
+            w[j] = 1 - (u - ind);
+        }
+        return;
+    }
     for (int j = 0; j < 3; ++j) {
         double x = pymod(pos[j] + R->half_width[j], R->width[j]) - R->half_step[j];
         double q = x / R->dxyz[j];

@@ -328,6 +328,7 @@ __device__ __forceinline__ void field_tape_body(const chromo_replica *R, const i
     for (int j = 0; j < CHROMO_MAX_BINDERS; ++j) S2[j] = 0;
     for (int j = 0; j < CHROMO_MAX_BINDERS * CHROMO_MAX_BINDERS; ++j) SX[j] = 0;
     const double V = R->bead_vol, vf = R->vf_limit, chi = R->chi;
+    const bool thr = R->fast_field == 0;
     for (int q = lane; q < U; q += WAVE) {
         const int s0 = A.uoff[q], s1 = A.uoff[q + 1];
         for (int a = s0 + 1; a < s1; ++a) {
@@ -347,12 +348,12 @@ __device__ __forceinline__ void field_tape_body(const chromo_replica *R, const i
             const double c0 = A.c0[e];
             const double pre = k ? 1. : -1.;
             const double t0 = pre * c0;
-            if (fabs(t0) > 1E-18) acc[0] += t0;
+            if (!thr || fabs(t0) > 1E-18) acc[0] += t0;
             for (int m = 1; m < Bp1; ++m) {
                 int st = (k == 0 || !state_change) ? A.stold[(m - 1) * cap + i]
                                                    : A.stnew[(m - 1) * cap + i];
                 double t = pre * (c0 * (double)st);
-                if (fabs(t) > 1E-18) acc[m] += t;
+                if (!thr || fabs(t) > 1E-18) acc[m] += t;
             }
         }
         int64_t v = A.uvox[q];

@@ -468,6 +468,7 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, double u, Pro
         double S2[NB], SX[NB * NB], NS = 0;
         for (int j = 0; j < NB; ++j) S2[j] = 0;
         for (int j = 0; j < NB * NB; ++j) SX[j] = 0;
+        const bool thr = R->fast_field == 0;
         for (int e = lane; e < E; e += WAVE) {
             const int hs = A.eslot[e];
             if (A.hmin[hs] != e) continue;
@@ -504,12 +505,12 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, double u, Pro
                     const double c0 = cv[u];
                     const double pre = ((ev[u] >> 3) & 1) ? 1. : -1.;
                     const double t0 = pre * c0;
-                    if (fabs(t0) > 1E-18) acc[0] += t0;
+                    if (!thr || fabs(t0) > 1E-18) acc[0] += t0;
 #pragma unroll
                     for (int m = 1; m <= NB; ++m) {
                         const int st = (sv[u] >> (8 * (m - 1))) & 0xff;
                         const double t = pre * (c0 * (double)st);
-                        if (fabs(t) > 1E-18) acc[m] += t;
+                        if (!thr || fabs(t) > 1E-18) acc[m] += t;
                     }
                 }
             }

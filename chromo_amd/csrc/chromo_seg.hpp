@@ -143,16 +143,17 @@ __device__ __forceinline__ bool axis_hit(int c, int d, int64_t nn)
 
 // One contribution of the per-voxel fold (fields.pyx:1491-1520); c0 is the corner
 // weight divided by vol_bin, sq the packed binder states of the position.
+// thr: the 1e-18 rules apply (not under the fast field, fields.pyx:1346-1366).
 template <int NB>
-__device__ __forceinline__ void fold_c0(double c0, double pre, int32_t sq, double *acc)
+__device__ __forceinline__ void fold_c0(double c0, double pre, int32_t sq, double *acc, bool thr = true)
 {
     const double t0 = pre * c0;
-    if (fabs(t0) > 1E-18) acc[0] += t0;
+    if (!thr || fabs(t0) > 1E-18) acc[0] += t0;
 #pragma unroll
     for (int m = 1; m <= NB; ++m) {
         const int st = (sq >> (8 * (m - 1))) & 0xff;
         const double t = pre * (c0 * (double)st);
-        if (fabs(t) > 1E-18) acc[m] += t;
+        if (!thr || fabs(t) > 1E-18) acc[m] += t;
     }
 }
 
@@ -392,6 +393,7 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
         const int nx = (int)R->nx, ny = (int)R->ny, nz = (int)R->nz;
         const double vol_bin = R->vol_bin;
         const double *avol = R->access_vol;
+        const bool thr = R->fast_field == 0;
         // positions 2i (current) and 2i + 1 (trial; the current one for a state change)
         if (act) {
 #pragma unroll
@@ -556,7 +558,7 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
                     const bool hx = vx == x0 || vx == x1, hy = vy == y0 || vy == y1, hz = vz == z0 || vz == z1;
                     if (hx && hy && hz) {
                         const int l = (vx == x1 ? 1 : 0) | (vy == y1 ? 2 : 0) | (vz == z1 ? 4 : 0);
-                        fold_c0<NB>(S.c0[p * 8 + l], pre, pr.w, acc);
+                        fold_c0<NB>(S.c0[p * 8 + l], pre, pr.w, acc, thr);
                     }
                 } else {
                     int dx = vx - x0, dy = vy - y0, dz = vz - z0;
@@ -567,7 +569,7 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
                         for (int l = 0; l < 8; ++l)
                             if (axis_hit(l & 1, dx, nx) && axis_hit((l >> 1) & 1, dy, ny) &&
                                 axis_hit((l >> 2) & 1, dz, nz))
-                                fold_c0<NB>(S.c0[p * 8 + l], pre, pr.w, acc);
+                                fold_c0<NB>(S.c0[p * 8 + l], pre, pr.w, acc, thr);
                 }
             }
             if (own) {

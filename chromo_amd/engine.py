@@ -67,6 +67,8 @@ class FieldSpec:
     confine_type: int = 0
     confine_length: float = 0.0
     access_vol: Optional[np.ndarray] = None   # (n_bins,) accessible volumes, None: vol_bin
+    fast_field: bool = False                  # sub-bin lookup weights in the move dE
+    n_points: int = 1000                      # sub-bins per voxel side (made even)
 
     @property
     def n_bins(self) -> int:
@@ -314,6 +316,15 @@ class DeviceReplicas:
                 R.confine_length = float(f.confine_length)
                 if av_list[i] is not None:
                     R.access_vol = base(self.access_vol) + 8 * int(self.o_av[i])
+                if f.fast_field:
+                    npts = int(f.n_points) + int(f.n_points) % 2      # init_fast_field (fields.pyx:610)
+                    if npts < 2:
+                        raise ValueError("fast_field needs n_points >= 1")
+                    R.fast_field = 1
+                    R.half_points = float(npts // 2)
+                    for j, (nn, d) in enumerate(zip((f.nx, f.ny, f.nz), (dx, dy, dz))):
+                        R.n_sub[j] = npts * int(nn)
+                        R.sub_width[j] = d / npts
             else:
                 R.nx = R.ny = R.nz = 1
             for a in range(B):

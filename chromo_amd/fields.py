@@ -108,8 +108,6 @@ class UniformDensityField(FieldBase):
             raise NotImplementedError(
                 "this build runs one polymer per field (the reference's field sums the "
                 "densities of all its polymers, fields.pyx:2063-2106)")
-        if fast_field:
-            raise NotImplementedError("fast_field sub-bin tables are not implemented")
         if confine_type not in _CONFINE:
             raise ValueError("Confinement type " + str(confine_type) + " not found.")
         self.x_width = float(x_width)
@@ -128,7 +126,9 @@ class UniformDensityField(FieldBase):
         self.vf_limit = float(np.float32(vf_limit))
         self.binder_dict = self.binders.to_dict(orient='records')
         self.fast_field = fast_field
-        self.n_points = n_points
+        # init_fast_field (fields.pyx:577-671) makes n_points even; the sub-bin
+        # weights are evaluated in the kernels from the same expressions
+        self.n_points = n_points + (n_points % 2) if fast_field else n_points
         self._engine = None
         self.update_all_densities_for_all_polymers()
         self.dict_ = self.get_dict()
@@ -216,6 +216,7 @@ class UniformDensityField(FieldBase):
                          chi=self.chi, bead_vol=float(bead_vol), pref=pref, intra=intra,
                          cross=cross, confine_type=_CONFINE[self.confine_type],
                          access_vol=self.access_vol_array(),
+                         fast_field=bool(self.fast_field), n_points=int(self.n_points),
                          confine_length=self.confine_length)
 
     def access_vol_array(self):

@@ -153,6 +153,13 @@ typedef struct {
     double F[CHROMO_MAX_BINDERS][CHROMO_MAX_MARK][CHROMO_MAX_SITES + 1];
     const double *access_vol;   /* n_bins accessible volumes (get_accessible_volumes,
                                    chromo/fields.pyx:714-770), or NULL: every voxel vol_bin */
+    int32_t fast_field;         /* 1: move dE from sub-bin weights (init_fast_field and
+                                   get_change_in_density_quickly, chromo/fields.pyx:577-671,
+                                   1235-1368): no 1e-18 rules in the fold */
+    int32_t pad_ff;
+    int64_t n_sub[3];           /* sub-bins per axis: n_points * n (n_points even) */
+    double sub_width[3];        /* voxel side / n_points */
+    double half_points;         /* n_points / 2 */
 } chromo_replica;
 
 /* ---- library ---------------------------------------------------------- */

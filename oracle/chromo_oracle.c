@@ -364,6 +364,9 @@ struct or_sim {
     int64_t *wrap;                 /* inds_xyz_to_super, (nx+1)(ny+1)(nz+1) */
     double *density, *density_trial;
     double *access_vols;           /* access_vols[bin] (fields.pyx:714-770) */
+    int fast_field;                /* init_fast_field (fields.pyx:577-671) */
+    int64_t n_points, n_sub[3];
+    double sub_w[3];
     int64_t *affected;
     int faithful;
     /* move scratch */
@@ -450,6 +453,15 @@ or_sim *or_new(const or_spec *sp, int32_t rng_mode, uint32_t seed, uint32_t stre
     s->dxyz[2] = sp->z_width / (double)s->nz;
     s->n_bins = s->nx * s->ny * s->nz;
     s->vol_bin = sp->x_width * sp->y_width * sp->z_width / (double)s->n_bins;
+    s->fast_field = sp->fast_field;
+    s->n_points = sp->n_points + (sp->n_points % 2);
+    {
+        const int64_t nn[3] = {s->nx, s->ny, s->nz};
+        for (int j = 0; j < 3; ++j) {
+            s->sub_w[j] = s->dxyz[j] / (double)s->n_points;
+            s->n_sub[j] = s->n_points * nn[j];
+        }
+    }
     s->access_vols = (double *)malloc((size_t)s->n_bins * sizeof(double));
     for (int64_t i = 0; i < s->n_bins; ++i)
         s->access_vols[i] = sp->access_vols ? sp->access_vols[i] : s->vol_bin;
@@ -1110,6 +1122,43 @@ static void bead_voxels(const or_sim *s, const double *pos, int64_t idx8[8], dou
     idx8[7] = wrap_index(s, 1 + ix[0], 1 + ix[1], 1 + ix[2]);
 }
 
+/* bead_voxels under the fast field: each axis' sub-bin floor((x + W/2) / sbw)
+ * mod (n_points n) (get_change_in_density_quickly, fields.pyx:1255-1300) and that
+ * sub-bin's lower voxel and weight as init_fast_field tabulates them
+ * (fields.pyx:614-671), evaluated from the same expressions. */
+static void bead_voxels_fast(const or_sim *s, const double *pos, int64_t idx8[8], double wt8[8])
+{
+    const int64_t nm1[3] = {s->nx - 1, s->ny - 1, s->nz - 1};
+    const double half = (double)(s->n_points / 2);
+    int64_t ix[3];
+    double w[3];
+    for (int j = 0; j < 3; ++j) {
+        double sb = floor((pos[j] + s->half_width[j]) / s->sub_w[j]);
+        int64_t i = (int64_t)sb % s->n_sub[j];
+        if (i < 0) i += s->n_sub[j];
+        double ind = floor((((double)i - half) * s->sub_w[j]) / s->dxyz[j]);
+        ix[j] = (ind == -1) ? nm1[j] : (int64_t)ind;
+        double upper = (((double)i - half) * s->sub_w[j]) / s->dxyz[j] - ind;
+        w[j] = 1 - upper;
+    }
+    wt8[0] = w[0] * w[1] * w[2];
+    wt8[1] = (1 - w[0]) * w[1] * w[2];
+    wt8[2] = w[0] * (1 - w[1]) * w[2];
+    wt8[3] = (1 - w[0]) * (1 - w[1]) * w[2];
+    wt8[4] = w[0] * w[1] * (1 - w[2]);
+    wt8[5] = (1 - w[0]) * w[1] * (1 - w[2]);
+    wt8[6] = w[0] * (1 - w[1]) * (1 - w[2]);
+    wt8[7] = (1 - w[0]) * (1 - w[1]) * (1 - w[2]);
+    idx8[0] = wrap_index(s, ix[0], ix[1], ix[2]);
+    idx8[1] = wrap_index(s, 1 + ix[0], ix[1], ix[2]);
+    idx8[2] = wrap_index(s, ix[0], 1 + ix[1], ix[2]);
+    idx8[3] = wrap_index(s, 1 + ix[0], 1 + ix[1], ix[2]);
+    idx8[4] = wrap_index(s, ix[0], ix[1], 1 + ix[2]);
+    idx8[5] = wrap_index(s, 1 + ix[0], ix[1], 1 + ix[2]);
+    idx8[6] = wrap_index(s, ix[0], 1 + ix[1], 1 + ix[2]);
+    idx8[7] = wrap_index(s, 1 + ix[0], 1 + ix[1], 1 + ix[2]);
+}
+
 /* get_change_in_density, fields.pyx:1370-1522.  Fills density_trial rows of the
  * touched voxels and s->bins in CPython set order; returns the bin count. */
 static int64_t change_in_density(or_sim *s, const double *r_trial, const int64_t *st_trial,
@@ -1122,9 +1171,14 @@ static int64_t change_in_density(or_sim *s, const double *r_trial, const int64_t
         int64_t b = inds[i];
         int64_t idx[2][8];
         double wt[2][8];
-        bead_voxels(s, s->r + 3 * b, idx[0], wt[0]);
-        if (state_change == 0) bead_voxels(s, r_trial + 3 * b, idx[1], wt[1]);
-        else {
+        if (s->fast_field) {
+            bead_voxels_fast(s, s->r + 3 * b, idx[0], wt[0]);
+            if (state_change == 0) bead_voxels_fast(s, r_trial + 3 * b, idx[1], wt[1]);
+        } else {
+            bead_voxels(s, s->r + 3 * b, idx[0], wt[0]);
+            if (state_change == 0) bead_voxels(s, r_trial + 3 * b, idx[1], wt[1]);
+        }
+        if (state_change != 0) {
             memcpy(idx[1], idx[0], sizeof(idx[0]));
             memcpy(wt[1], wt[0], sizeof(wt[0]));
         }
@@ -1142,16 +1196,18 @@ static int64_t change_in_density(or_sim *s, const double *r_trial, const int64_t
             for (int l = 0; l < 8; ++l) {
                 int64_t bin = idx[k][l];
                 double *row = s->density_trial + bin * Bp1;
+                /* the fast path (fields.pyx:1346-1366) keeps every contribution */
+                const int thr = !s->fast_field;
                 if (pyset_contains(&s->found, bin)) {
                     for (int64_t m = 0; m < Bp1; ++m) {
                         double temp = pre * dt[k][m][l];
-                        if (fabs(temp) > 1E-18) row[m] += temp;
+                        if (!thr || fabs(temp) > 1E-18) row[m] += temp;
                     }
                 } else {
                     pyset_add(&s->found, bin);
                     for (int64_t m = 0; m < Bp1; ++m) {
                         double temp = pre * dt[k][m][l];
-                        row[m] = (fabs(temp) > 1E-18) ? temp : 0;
+                        row[m] = (!thr || fabs(temp) > 1E-18) ? temp : 0;
                     }
                 }
             }

@@ -45,6 +45,8 @@ typedef struct {
     int32_t confine_type;               /* 0 none, 1 spherical, 2 cubical */
     double confine_length;
     const double *access_vols;          /* n_bins accessible volumes (copied), or NULL: vol_bin */
+    int32_t fast_field;                 /* 1: sub-bin lookup weights in the move dE */
+    int64_t n_points;                   /* sub-bins per voxel side (made even) */
 } or_spec;
 
 typedef struct {

@@ -41,6 +41,7 @@ class OrSpec(ctypes.Structure):
         ("intra", ctypes.c_double * OR_MAXB), ("cross", ctypes.c_double * (OR_MAXB * OR_MAXB)),
         ("confine_type", ctypes.c_int32), ("confine_length", ctypes.c_double),
         ("access_vols", _dp),
+        ("fast_field", ctypes.c_int32), ("n_points", ctypes.c_int64),
     ]
 
 
@@ -198,6 +199,8 @@ class OracleSim:
                 sp.cross[a * B + b] = float(cross[a, b])
         sp.confine_type = int(field.confine_type)
         sp.confine_length = float(field.confine_length)
+        sp.fast_field = 1 if getattr(field, "fast_field", False) else 0
+        sp.n_points = int(getattr(field, "n_points", 1000))
         av = getattr(field, "access_vol", None)
         if av is not None and field.field_on:
             av = _d(av).reshape(-1)

@@ -279,3 +279,28 @@ def test_accessible_volumes_spherical_confinement(oracle):
         assert np.array_equal(delta_g, delta_o[bins_g])
         assert _dE_close(np.array([dEg]), np.array([dEo]))[0]
     _run_pair(oracle, s, 8, waves=2, rel=1e-9)
+
+
+@pytest.mark.parametrize("npts,waves", [(1000, 2), (6, 1), (6, 2)])
+def test_fast_field(oracle, npts, waves):
+    """fast_field = 1: the move dE from sub-bin weights (chromo/fields.pyx:577-671,
+    1235-1368) with no 1e-18 rules, the stored densities from the exact scatter.
+    Tape delta densities bit for bit, then a run of every move type."""
+    say("fast field", npts, waves)
+    s = W.make_system(1500, 3, 16, seed=66, eps=[-4, -3, -2], cross_eps=0.5, mu=[-0.4, 0.2, -1])
+    s.field.fast_field, s.field.n_points = True, npts
+    o = oracle.OracleSim(s.chain, s.field)
+    dr = _dev([s])
+    assert np.array_equal(dr.density_of(0), o.density())
+    rs = np.random.RandomState(4)
+    for n in (1, 12, 45):
+        ind0 = rs.randint(0, 1500 - n)
+        inds = np.arange(ind0, ind0 + n)
+        rt = s.chain.r[inds] + rs.normal(scale=8.0, size=(n, 3))
+        st = s.chain.states[inds]
+        dEo, bins_o, delta_o = o.field_dE_tape(inds, rt, st, 0)
+        dEg, bins_g, delta_g = dr.field_dE_tape(0, inds, rt, st.astype(np.int8), False)
+        assert sorted(bins_g) == sorted(bins_o)
+        assert np.array_equal(delta_g, delta_o[bins_g])
+        assert _dE_close(np.array([dEg]), np.array([dEo]))[0]
+    _run_pair(oracle, s, 8, waves=waves)

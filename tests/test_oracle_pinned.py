@@ -16,6 +16,7 @@ import pytest
 
 from chromo_amd.engine import ChainSpec, FieldSpec
 from chromo_amd import workloads as W
+from test_oracle_energy import _density_bruteforce, _field_energy
 
 
 def _perp_unit(t3):
@@ -358,3 +359,40 @@ def test_field_dE_with_accessible_volumes_bruteforce(oracle, seed):
         phi = rho[:, 0] * s.field.bead_vol
         return E + np.sum(s.field.chi * (av / s.field.bead_vol) * phi ** 2)
     assert abs(dE - (energy(rho1) - energy(rho0))) < 1e-8 * max(1, abs(energy(rho1)))
+
+
+# ------------------------------------------------------------ fast field ---
+
+def _snap(field, r, npts):
+    """A position's sub-bin lower edge: the point whose exact trilinear weights
+    the fast field's tables hold (fields.pyx:614-671, 1255-1300)."""
+    Wd = np.array([field.x_width, field.y_width, field.z_width])
+    n = np.array([field.nx, field.ny, field.nz])
+    sbw = (Wd / n) / npts
+    return np.floor((r + Wd / 2) / sbw) * sbw - Wd / 2
+
+
+@pytest.mark.parametrize("npts", [4, 10, 1000])
+def test_fast_field_weights_are_sub_bin_edge_weights(oracle, npts):
+    """Under fast_field the move's delta density is the difference of the exact
+    trilinear weights at the moved beads' sub-bin lower edges (current and
+    trial), with no 1e-18 rules; the stored densities stay the exact scatter."""
+    s = W.make_system(60, 2, 5, seed=3, eps=[-3.0, -1.5], cross_eps=0.8, chi=1.3)
+    rs = np.random.RandomState(npts)
+    st = rs.randint(0, 3, size=(60, 2))
+    s.chain.states = st
+    s.field.fast_field, s.field.n_points = True, npts
+    sim = oracle.OracleSim(s.chain, s.field)
+    rho0 = sim.density()
+    inds = np.arange(20, 27)
+    rt = s.chain.r[inds] + rs.normal(scale=11.0, size=(7, 3))
+    dE, bins, delta = sim.field_dE_tape(inds, rt, st[inds], 0)
+    d_old, _ = _density_bruteforce(s.field, _snap(s.field, s.chain.r[inds], npts), st[inds])
+    d_new, vb = _density_bruteforce(s.field, _snap(s.field, rt, npts), st[inds])
+    assert np.allclose(delta, d_new - d_old, rtol=1e-10, atol=1e-13)
+    E0 = _field_energy(s.field, rho0, st, 2, vb)
+    E1 = _field_energy(s.field, rho0 + delta, st, 2, vb)
+    assert abs(dE - (E1 - E0)) < 1e-8 * max(1, abs(E1))
+    # the weights differ from the exact ones by less than one sub-bin per axis
+    d_exact_new, _ = _density_bruteforce(s.field, rt, st[inds])
+    assert np.max(np.abs(d_new - d_exact_new)) * vb <= 7 * 3.0 / npts + 1e-12
