@@ -38,6 +38,7 @@ ERROR_TEXT = {
     4: "binder count outside 1..4",
     5: "a rejection-sampling or probe loop exceeded its iteration bound (amplitude below 1?)",
     6: "internal move-arena index check failed",
+    7: "a twisting chain (SSTWLC) runs with one wavefront per replica (waves=1)",
 }
 
 
@@ -87,12 +88,13 @@ class ChromoReplica(ctypes.Structure):
         ("fast_field", ctypes.c_int32), ("pad_ff", ctypes.c_int32),
         ("n_sub", ctypes.c_int64 * 3), ("sub_width", ctypes.c_double * 3),
         ("half_points", ctypes.c_double),
+        ("twist", ctypes.c_int32), ("pad_tw", ctypes.c_int32),
     ]
 
 
 class ChromoEnergyModel(ctypes.Structure):
     _fields_ = [
-        ("N", ctypes.c_int64), ("B", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("N", ctypes.c_int64), ("B", ctypes.c_int32), ("twist", ctypes.c_int32),
         ("t3", ctypes.c_void_p), ("states", ctypes.c_void_p), ("marks", ctypes.c_void_p),
         ("bonds", ctypes.c_void_p),
         ("nx", ctypes.c_int64), ("ny", ctypes.c_int64), ("nz", ctypes.c_int64),

@@ -111,7 +111,7 @@ __device__ __forceinline__ int big_corner(const int4 &pr, int l, int nx, int ny)
 // (SSWLC.compute_dE, polymers.pyx:1024-1082), field dE
 // (UniformDensityField.compute_dE, fields.pyx:1149-1233), Metropolis with u
 // (mc_sim.pyx:152-182).  Writes nothing outside S.  Called by one whole wavefront.
-template <int NB>
+template <int NB, bool TW = false>
 __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal &P, double u, const BigLds &S,
                                          SegRes &res)
 {
@@ -173,7 +173,9 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
                 t30[j] = Pp[(3 + j) * N + ind0 - 1];
             }
             Bond bd = load_bond(R->bonds, N, ind0 - 1);
-            ef = pair_dE_forward(bd, r0, &o[0], &w[0], t30, &o[3], &w[3]);
+            double twt = 0, twc = 0;
+            if constexpr (TW) twist_forward(R->bonds, R->pos, N, ind0 - 1, o, w, twt, twc);
+            ef = pair_dE_forward(bd, r0, &o[0], &w[0], t30, &o[3], &w[3], twt, twc);
         }
         if (act && i == n - 1 && indf != N) {
             double r1[3], t31[3];
@@ -182,7 +184,9 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
                 t31[j] = Pp[(3 + j) * N + indf];
             }
             Bond bd = load_bond(R->bonds, N, indf - 1);
-            er = pair_dE_reverse(bd, &o[0], &w[0], r1, &o[3], &w[3], t31);
+            double twt = 0, twc = 0;
+            if constexpr (TW) twist_reverse(R->bonds, R->pos, N, indf, o, w, twt, twc);
+            er = pair_dE_reverse(bd, &o[0], &w[0], r1, &o[3], &w[3], t31, twt, twc);
         }
         if (c == 0 && ind0 != 0) ef = readlane_f64(ef, 0);
         if (c == nchunks - 1 && indf != N) er = readlane_f64(er, (int)((n - 1) % BIG_CHUNK));

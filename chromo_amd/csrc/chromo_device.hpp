@@ -40,6 +40,7 @@ enum : int {
     ERR_NBINDERS = 4,     // binding move on a chain without binders
     ERR_SPIN = 5,         // a rejection or probe loop ran past SPIN_LIMIT iterations
     ERR_ARENA = 6,        // a move-arena index check failed (internal error)
+    ERR_TWIST_WAVES = 7,  // a twisting chain in the speculative (two-wavefront) kernel
 };
 
 // Iteration bound of every data-dependent loop (rejection sampling, distinct-bead
@@ -718,10 +719,60 @@ __device__ __forceinline__ double E_pair(const Bond &bd, const double *bend, dou
     return 0.5 * bd.eb * dot3(bend, bend) + 0.5 * bd.ep * (q * q) + 0.5 * bd.eperp * dot3(perp, perp);
 }
 
-// bead_pair_dE_poly_forward, polymers.pyx:1177-1277
+// ---- twist (SSTWLC, chromo/polymers.pyx:1889-2320) ----------------------------
+//
+// The twist rows of a twisting chain's bond table: 5 eps_twist = lt / (delta lp),
+// 6 the natural twist of the bond, bead_length * (2 pi / 10.5) / 0.332.
+
+__device__ __forceinline__ void load_twist(const double *bonds_, int64_t N, int64_t b, double &et, double &tw0)
+{
+    const GLOBAL_AS double *bonds = gmem(bonds_);
+    const int64_t M = N - 1;
+    et = bonds[5 * M + b];
+    tw0 = bonds[6 * M + b];
+}
+
+// compute_twist_angle_omega (chromo/polymers.pyx:3427-3461): with t1 = t2 x t3,
+// omega = atan2(t2_0 . t1_1 - t1_0 . t2_1, t1_0 . t1_1 + t2_0 . t2_1).
+__device__ __forceinline__ double twist_omega(const double *t2_0, const double *t3_0, const double *t2_1,
+                                              const double *t3_1)
+{
+    const double t1_0[3] = {t2_0[1] * t3_0[2] - t2_0[2] * t3_0[1], t2_0[2] * t3_0[0] - t2_0[0] * t3_0[2],
+                            t2_0[0] * t3_0[1] - t2_0[1] * t3_0[0]};
+    const double t1_1[3] = {t2_1[1] * t3_1[2] - t2_1[2] * t3_1[1], t2_1[2] * t3_1[0] - t2_1[0] * t3_1[2],
+                            t2_1[0] * t3_1[1] - t2_1[1] * t3_1[0]};
+    return atan2(dot3(t2_0, t1_1) - dot3(t1_0, t2_1), dot3(t1_0, t1_1) + dot3(t2_0, t2_1));
+}
+
+// Python's float floor division a // b (b > 0): the exact remainder fmod(a, b),
+// the quotient (a - mod) / b, one step down when the remainder has the other sign,
+// then the nearest integer of that quotient.
+__device__ __forceinline__ double py_floordiv(double a, double b)
+{
+    double mod = fmod(a, b);
+    double div = (a - mod) / b;
+    if (mod != 0 && ((b < 0) != (mod < 0))) div -= 1.0;
+    if (div == 0) return copysign(0.0, a / b);
+    double fl = floor(div);
+    if (div - fl > 0.5) fl += 1.0;
+    return fl;
+}
+
+// The twist term of E_pair_with_twist (polymers.pyx:2072-2082): the twist angle's
+// departure from the natural twist, wrapped into [-pi, pi).
+__device__ __forceinline__ double twist_energy(double et, double tw0, double omega)
+{
+    const double two_pi = 2 * M_PI;
+    double d = omega - tw0;
+    d -= two_pi * py_floordiv(d + M_PI, two_pi);
+    return 0.5 * et * (d * d);
+}
+
+// bead_pair_dE_poly_forward, polymers.pyx:1177-1277; tw_t / tw_c: the twist terms
+// of E_pair_with_twist for a twisting chain (polymers.pyx:2123-2175), else 0.
 __device__ __forceinline__ double pair_dE_forward(const Bond &bd, const double *r0, const double *r1,
                                   const double *tr1, const double *t30, const double *t31,
-                                  const double *tt31)
+                                  const double *tt31, double tw_t = 0, double tw_c = 0)
 {
     double dr_t[3], dr[3], drp_t[3], drp[3], be_t[3], be[3];
     for (int i = 0; i < 3; ++i) {
@@ -735,13 +786,13 @@ __device__ __forceinline__ double pair_dE_forward(const Bond &bd, const double *
         be_t[i] = tt31[i] - t30[i] - drp_t[i] * bd.eta;
         be[i] = t31[i] - t30[i] - drp[i] * bd.eta;
     }
-    return E_pair(bd, be_t, par_t, drp_t) - E_pair(bd, be, par, drp);
+    return (E_pair(bd, be_t, par_t, drp_t) + tw_t) - (E_pair(bd, be, par, drp) + tw_c);
 }
 
-// bead_pair_dE_poly_reverse, polymers.pyx:1279-1346
+// bead_pair_dE_poly_reverse, polymers.pyx:1279-1346 (twist: polymers.pyx:2177-2248)
 __device__ __forceinline__ double pair_dE_reverse(const Bond &bd, const double *r0, const double *tr0,
                                   const double *r1, const double *t30, const double *tt30,
-                                  const double *t31)
+                                  const double *t31, double tw_t = 0, double tw_c = 0)
 {
     double dr_t[3], dr[3], drp_t[3], drp[3], be_t[3], be[3];
     for (int i = 0; i < 3; ++i) {
@@ -755,7 +806,38 @@ __device__ __forceinline__ double pair_dE_reverse(const Bond &bd, const double *
         be_t[i] = t31[i] - tt30[i] - drp_t[i] * bd.eta;
         be[i] = t31[i] - t30[i] - drp[i] * bd.eta;
     }
-    return E_pair(bd, be_t, par_t, drp_t) - E_pair(bd, be, par, drp);
+    return (E_pair(bd, be_t, par_t, drp_t) + tw_t) - (E_pair(bd, be, par, drp) + tw_c);
+}
+
+// Twist terms of the forward pair (bead a = b0 - 1 fixed, bead b0 moved: its
+// current frame o[3..8], trial w[3..8]) and of the reverse pair (bead indf - 1
+// moved, bead a = indf fixed), with bead a's t3 / t2 read from the chain.
+__device__ __forceinline__ void twist_forward(const double *bonds, const double *pos, int64_t N, int64_t a,
+                                              const double *o, const double *w, double &tw_t, double &tw_c)
+{
+    double et, tw0, t3a[3], t2a[3];
+    load_twist(bonds, N, a, et, tw0);
+    const GLOBAL_AS double *P = gmem(pos);
+    for (int j = 0; j < 3; ++j) {
+        t3a[j] = P[(3 + j) * N + a];
+        t2a[j] = P[(6 + j) * N + a];
+    }
+    tw_c = twist_energy(et, tw0, twist_omega(t2a, t3a, &o[6], &o[3]));
+    tw_t = twist_energy(et, tw0, twist_omega(t2a, t3a, &w[6], &w[3]));
+}
+
+__device__ __forceinline__ void twist_reverse(const double *bonds, const double *pos, int64_t N, int64_t a,
+                                              const double *o, const double *w, double &tw_t, double &tw_c)
+{
+    double et, tw0, t3a[3], t2a[3];
+    load_twist(bonds, N, a - 1, et, tw0);
+    const GLOBAL_AS double *P = gmem(pos);
+    for (int j = 0; j < 3; ++j) {
+        t3a[j] = P[(3 + j) * N + a];
+        t2a[j] = P[(6 + j) * N + a];
+    }
+    tw_c = twist_energy(et, tw0, twist_omega(&o[6], &o[3], t2a, t3a));
+    tw_t = twist_energy(et, tw0, twist_omega(&w[6], &w[3], t2a, t3a));
 }
 
 // bead_binding_dE, polymers.pyx:1408-1538, with host-built -log(single-site Z)

@@ -51,6 +51,7 @@ int eset(int code, const std::string &m) { return chromo::set_last_error(code, m
 struct EModel {
     int64_t N;
     int B;
+    int twist;
     GridGeom G;
     const double *t3;
     const int8_t *states, *marks;
@@ -459,7 +460,8 @@ __global__ void k_ref_voxel(EModel M, const double *pref, double chi, const doub
     block_reduce_store(acc, nt, part);
 }
 
-// slots: [0] SSWLC.compute_E bonds, [1..1+B) doubly bound counts
+// slots: [0] SSWLC.compute_E bonds (SSTWLC.compute_E with the twist term), [1..1+B)
+// doubly bound counts
 __global__ void k_ref_bead(EModel M, const double *pos, int64_t sc, int64_t sb, double *part, int nt)
 {
     const int B = M.B;
@@ -484,7 +486,17 @@ __global__ void k_ref_bead(EModel M, const double *pos, int64_t sc, int64_t sb, 
                 be[j] = t31[j];
                 be[j] += -t30[j] - bd.eta * drp[j];
             }
-            acc[0] += E_pair(bd, be, par, drp);
+            double tw = 0;
+            if (M.twist) {
+                double t20[3], t21[3], et, tw0;
+                for (int j = 0; j < 3; ++j) {
+                    t20[j] = pos[(6 + j) * sc + b * sb];
+                    t21[j] = pos[(6 + j) * sc + (b + 1) * sb];
+                }
+                load_twist(M.bonds, N, b, et, tw0);
+                tw = twist_energy(et, tw0, twist_omega(t20, t30, t21, t31));
+            }
+            acc[0] += E_pair(bd, be, par, drp) + tw;
         }
         for (int j = 0; j < B; ++j) acc[1 + j] += (M.states[(int64_t)j * N + b] == 2);
     }
@@ -527,6 +539,7 @@ int make_model(const chromo_energy_model *m, EModel &M)
     if (n_theta(m->B) > MAXT || 2 + 2 * m->B > MAXT) return eset(-1, "too many terms");
     M.N = m->N;
     M.B = m->B;
+    M.twist = m->twist != 0;
     M.G.nx = m->nx;
     M.G.ny = m->ny;
     M.G.nz = m->nz;
@@ -661,6 +674,7 @@ int chromo_energy_adjoint(const chromo_energy_model *model, const double *d_r, c
     EModel M;
     int rc = make_model(model, M);
     if (rc) return rc;
+    if (M.twist) return eset(-1, "the energy adjoint has no twist term (SSTWLC)");
     if (!d_r || !d_theta || !d_E) return eset(-1, "NULL argument");
     hipStream_t st = (hipStream_t)stream;
     EnergyBufs Bf;
@@ -678,6 +692,7 @@ int chromo_energy_adjoint_noalloc(const chromo_energy_model *model, const double
     EModel M;
     int rc = make_model(model, M);
     if (rc) return rc;
+    if (M.twist) return eset(-1, "the energy adjoint has no twist term (SSTWLC)");
     if (!d_r || !d_theta || !d_E) return eset(-1, "NULL argument");
     EnergyBufs Bf;
     const size_t bytes = energy_carve(M, d_grad_r != nullptr, false, nullptr, Bf);

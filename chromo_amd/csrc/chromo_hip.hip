@@ -53,12 +53,22 @@ __global__ void __launch_bounds__(64) k_mc_run(const chromo_replica *__restrict_
         for (int i = 0; i < nc; ++i) sctrl[i] = R->ctrl[i];
     __syncthreads();
     int err = 0;
-    switch (R->B) {
-    case 1: run_steps<1>(R, rng, K, sctrl, nc, steps, err, G, lds_dyn); break;
-    case 2: run_steps<2>(R, rng, K, sctrl, nc, steps, err, G, lds_dyn); break;
-    case 3: run_steps<3>(R, rng, K, sctrl, nc, steps, err, G, lds_dyn); break;
-    case 4: run_steps<4>(R, rng, K, sctrl, nc, steps, err, G, lds_dyn); break;
-    default: err = ERR_NBINDERS; break;
+    if (R->twist) {
+        switch (R->B) {
+        case 1: run_steps<1, true>(R, rng, K, sctrl, nc, steps, err, G, lds_dyn); break;
+        case 2: run_steps<2, true>(R, rng, K, sctrl, nc, steps, err, G, lds_dyn); break;
+        case 3: run_steps<3, true>(R, rng, K, sctrl, nc, steps, err, G, lds_dyn); break;
+        case 4: run_steps<4, true>(R, rng, K, sctrl, nc, steps, err, G, lds_dyn); break;
+        default: err = ERR_NBINDERS; break;
+        }
+    } else {
+        switch (R->B) {
+        case 1: run_steps<1>(R, rng, K, sctrl, nc, steps, err, G, lds_dyn); break;
+        case 2: run_steps<2>(R, rng, K, sctrl, nc, steps, err, G, lds_dyn); break;
+        case 3: run_steps<3>(R, rng, K, sctrl, nc, steps, err, G, lds_dyn); break;
+        case 4: run_steps<4>(R, rng, K, sctrl, nc, steps, err, G, lds_dyn); break;
+        default: err = ERR_NBINDERS; break;
+        }
     }
     rng_store(R, rng, need_mt);
     if (lane_id() == 0)
@@ -90,12 +100,22 @@ __global__ void __launch_bounds__(64) k_mc_step(const chromo_replica *__restrict
     Counters K;
     counters_load(R, K);
     MoveOut o;
-    switch (R->B) {
-    case 1: mc_step_dev<1>(R, rng, C, o, G, lds_dyn, 1); break;
-    case 2: mc_step_dev<2>(R, rng, C, o, G, lds_dyn, 1); break;
-    case 3: mc_step_dev<3>(R, rng, C, o, G, lds_dyn, 1); break;
-    case 4: mc_step_dev<4>(R, rng, C, o, G, lds_dyn, 1); break;
-    default: o.err = ERR_NBINDERS; o.n = 0; break;
+    if (R->twist) {
+        switch (R->B) {
+        case 1: mc_step_dev<1, true>(R, rng, C, o, G, lds_dyn, 1); break;
+        case 2: mc_step_dev<2, true>(R, rng, C, o, G, lds_dyn, 1); break;
+        case 3: mc_step_dev<3, true>(R, rng, C, o, G, lds_dyn, 1); break;
+        case 4: mc_step_dev<4, true>(R, rng, C, o, G, lds_dyn, 1); break;
+        default: o.err = ERR_NBINDERS; o.n = 0; break;
+        }
+    } else {
+        switch (R->B) {
+        case 1: mc_step_dev<1>(R, rng, C, o, G, lds_dyn, 1); break;
+        case 2: mc_step_dev<2>(R, rng, C, o, G, lds_dyn, 1); break;
+        case 3: mc_step_dev<3>(R, rng, C, o, G, lds_dyn, 1); break;
+        case 4: mc_step_dev<4>(R, rng, C, o, G, lds_dyn, 1); break;
+        default: o.err = ERR_NBINDERS; o.n = 0; break;
+        }
     }
     if (!o.err && o.n > 0) record(R, C, o, K);
     rng_store(R, rng, need_mt);

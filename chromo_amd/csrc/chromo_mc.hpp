@@ -207,7 +207,7 @@ __device__ __forceinline__ void arena_sync()
     else wave_sync_global();
 }
 
-template <int KIND, int NB>
+template <int KIND, int NB, bool TW = false>
 __device__ __forceinline__ void move_body(const chromo_replica *R, double u, Proposal &P, MoveOut &out)
 {
     const int lane = lane_id();
@@ -297,7 +297,9 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, double u, Pro
                     t30[j] = Pp[(3 + j) * N + b - 1];
                 }
                 Bond bd = load_bond(R->bonds, N, b - 1);
-                e += pair_dE_forward(bd, r0, &o[0], &w[0], t30, &o[3], &w[3]);
+                double twt = 0, twc = 0;
+                if constexpr (TW) twist_forward(R->bonds, R->pos, N, b - 1, o, w, twt, twc);
+                e += pair_dE_forward(bd, r0, &o[0], &w[0], t30, &o[3], &w[3], twt, twc);
             }
             if (b + 1 != N) {
                 double r1[3], t31[3];
@@ -306,7 +308,9 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, double u, Pro
                     t31[j] = Pp[(3 + j) * N + b + 1];
                 }
                 Bond bd = load_bond(R->bonds, N, b);
-                e += pair_dE_reverse(bd, &o[0], &w[0], r1, &o[3], &w[3], t31);
+                double twt = 0, twc = 0;
+                if constexpr (TW) twist_reverse(R->bonds, R->pos, N, b + 1, o, w, twt, twc);
+                e += pair_dE_reverse(bd, &o[0], &w[0], r1, &o[3], &w[3], t31, twt, twc);
             }
             A.epart[i] = e;
         } else if (mv == CHROMO_CHANGE_BINDING_STATE) {
@@ -321,7 +325,9 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, double u, Pro
                     t30[j] = Pp[(3 + j) * N + ind0 - 1];
                 }
                 Bond bd = load_bond(R->bonds, N, ind0 - 1);
-                A.epart[0] = pair_dE_forward(bd, r0, &o[0], &w[0], t30, &o[3], &w[3]);
+                double twt = 0, twc = 0;
+                if constexpr (TW) twist_forward(R->bonds, R->pos, N, ind0 - 1, o, w, twt, twc);
+                A.epart[0] = pair_dE_forward(bd, r0, &o[0], &w[0], t30, &o[3], &w[3], twt, twc);
             }
             if (i == n - 1 && indf != N) {
                 double r1[3], t31[3];
@@ -330,7 +336,9 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, double u, Pro
                     t31[j] = Pp[(3 + j) * N + indf];
                 }
                 Bond bd = load_bond(R->bonds, N, indf - 1);
-                A.epart[1] = pair_dE_reverse(bd, &o[0], &w[0], r1, &o[3], &w[3], t31);
+                double twt = 0, twc = 0;
+                if constexpr (TW) twist_reverse(R->bonds, R->pos, N, indf, o, w, twt, twc);
+                A.epart[1] = pair_dE_reverse(bd, &o[0], &w[0], r1, &o[3], &w[3], t31, twt, twc);
             }
         }
         if (check_field) {
@@ -624,7 +632,7 @@ __device__ __forceinline__ void load_pos3(const double *Pp_, int64_t N, int64_t 
 // G: the caller's SEG_MAX evaluator block.  wide: the start of an LDS region that
 // holds a SEG_WIDE block (nullptr if none) and overlaps nstd SEG_MAX blocks, whose
 // hashes are emptied again after it is used.
-template <int NB>
+template <int NB, bool TW = false>
 __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rng, chromo_ctrl &C, MoveOut &out,
                                             const SegLds &G, char *wide, int nstd)
 {
@@ -857,7 +865,7 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
         sr.overflow = 1;
         if (n <= SEG_MAX) {
             SegKeep<NB> K;
-            seg_eval<NB>(R, P, u, G, sr, K);
+            seg_eval<NB, SEG_MAX, TW>(R, P, u, G, sr, K);
             if (!sr.overflow) {
                 if (sr.accepted) seg_commit<NB>(R, P.move, n, P.binder, P.mynew, K, G, sr.U);
                 wave_sync_global();
@@ -866,7 +874,7 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
             const SegLds GW = seg_lds<SEG_WIDE>(wide, NB + 1);
             seg_clear<SEG_WIDE>(GW);
             SegKeep<NB> K;
-            seg_eval<NB, SEG_WIDE>(R, P, u, GW, sr, K);
+            seg_eval<NB, SEG_WIDE, TW>(R, P, u, GW, sr, K);
             if (!sr.overflow) {
                 if (sr.accepted) seg_commit<NB>(R, P.move, n, P.binder, P.mynew, K, GW, sr.U);
                 wave_sync_global();
@@ -877,7 +885,7 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
         } else if (wide != nullptr && big_move(P.move)) {
             const BigLds GB = big_lds(wide, NB + 1, wide_lds_bytes(NB + 1));
             if (GB.ucap > 0) {
-                big_eval<NB>(R, P, u, GB, sr);
+                big_eval<NB, TW>(R, P, u, GB, sr);
                 if (!sr.overflow) {
                     if (sr.accepted) big_commit<NB>(R, P, GB, sr.U);
                     wave_sync_global();
@@ -901,7 +909,7 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
                 err = ERR_SEGCAP;
                 goto fail;
             }
-            move_body<1, NB>(R, u, P, out);
+            move_body<1, NB, TW>(R, u, P, out);
         }
     }
     if (out.err) {
@@ -1098,7 +1106,7 @@ __device__ __forceinline__ bool any_binding(const chromo_replica *R)
 // Kernels
 // ---------------------------------------------------------------------------
 
-template <int NB>
+template <int NB, bool TW = false>
 __device__ __forceinline__ void run_steps(const chromo_replica *R, WaveRng &rng, Counters &K, chromo_ctrl *sctrl,
                           int nc, int64_t steps, int &err, const SegLds &G, char *wide)
 {
@@ -1108,7 +1116,7 @@ __device__ __forceinline__ void run_steps(const chromo_replica *R, WaveRng &rng,
             if (c.on == 1) {
                 for (int j = 0; j < c.num_per_cycle; ++j) {
                     MoveOut o;
-                    mc_step_dev<NB>(R, rng, c, o, G, wide, 1);
+                    mc_step_dev<NB, TW>(R, rng, c, o, G, wide, 1);
                     if (o.err) {
                         err = o.err;
                         break;

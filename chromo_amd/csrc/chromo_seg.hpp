@@ -317,7 +317,7 @@ __device__ __forceinline__ void seg_r3(const SegBeads<NB> &B, int64_t x, int64_t
 // with the uniform draw u (mc_sim.pyx:152-182).  Writes nothing outside S and K.
 // Called by one whole wavefront; S's hash is empty on entry and on return.  B
 // holds what seg_load fetched for this move.
-template <int NB, int SMAX = SEG_MAX>
+template <int NB, int SMAX = SEG_MAX, bool TW = false>
 __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const Proposal &P, double u,
                                                 const SegLds &S, SegRes &res, SegKeep<NB> &K,
                                                 const SegBeads<NB> &B)
@@ -358,8 +358,13 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
     if (tangent || state_change) {
         double e = 0;
         if (act && tangent) {
-            if (B.hasl) e += pair_dE_forward(B.lb, &B.lr[0], &o[0], &w[0], &B.lr[3], &o[3], &w[3]);
-            if (B.hasr) e += pair_dE_reverse(B.rb, &o[0], &w[0], &B.rr[0], &o[3], &w[3], &B.rr[3]);
+            double tlt = 0, tlc = 0, trt = 0, trc = 0;
+            if constexpr (TW) {
+                if (B.hasl) twist_forward(R->bonds, R->pos, N, B.b - 1, o, w, tlt, tlc);
+                if (B.hasr) twist_reverse(R->bonds, R->pos, N, B.b + 1, o, w, trt, trc);
+            }
+            if (B.hasl) e += pair_dE_forward(B.lb, &B.lr[0], &o[0], &w[0], &B.lr[3], &o[3], &w[3], tlt, tlc);
+            if (B.hasr) e += pair_dE_reverse(B.rb, &o[0], &w[0], &B.rr[0], &o[3], &w[3], &B.rr[3], trt, trc);
         } else if (act) {
             e = bead_binding_dE<NB>(R, B.mods, so, sn);
         }
@@ -373,14 +378,18 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
 #pragma unroll
             for (int j = 0; j < 6; ++j) r0[j] = nb[j];
             const Bond bd{nb[6], nb[7], nb[8], nb[9], nb[10]};
-            ef = pair_dE_forward(bd, &r0[0], &o[0], &w[0], &r0[3], &o[3], &w[3]);
+            double twt = 0, twc = 0;
+            if constexpr (TW) twist_forward(R->bonds, R->pos, N, ind0 - 1, o, w, twt, twc);
+            ef = pair_dE_forward(bd, &r0[0], &o[0], &w[0], &r0[3], &o[3], &w[3], twt, twc);
         }
         if (lane == n - 1 && indf != N) {
             double r1[6];
 #pragma unroll
             for (int j = 0; j < 6; ++j) r1[j] = nb[11 + j];
             const Bond bd{nb[17], nb[18], nb[19], nb[20], nb[21]};
-            er = pair_dE_reverse(bd, &o[0], &w[0], &r1[0], &o[3], &w[3], &r1[3]);
+            double twt = 0, twc = 0;
+            if constexpr (TW) twist_reverse(R->bonds, R->pos, N, indf, o, w, twt, twc);
+            er = pair_dE_reverse(bd, &o[0], &w[0], &r1[0], &o[3], &w[3], &r1[3], twt, twc);
         }
         if (ind0 != 0) dEp += readlane_f64(ef, 0);
         if (indf != N) dEp += readlane_f64(er, (int)(n - 1));
@@ -614,13 +623,13 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
 }
 
 // seg_eval_loaded with its own loads (the one-at-a-time path).
-template <int NB, int SMAX = SEG_MAX>
+template <int NB, int SMAX = SEG_MAX, bool TW = false>
 __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal &P, double u, const SegLds &S,
                                          SegRes &res, SegKeep<NB> &K)
 {
     SegBeads<NB> B;
     seg_load<NB>(R, P.move, P.n, P.ind0, P.indf, P.mypick, B);
-    seg_eval_loaded<NB, SMAX>(R, P, u, S, res, K, B);
+    seg_eval_loaded<NB, SMAX, TW>(R, P, u, S, res, K, B);
 }
 
 // Writes an accepted proposal: bead coordinates or states (MCAdapter.accept,

@@ -30,6 +30,12 @@ k_mc_spec(const chromo_replica *__restrict__ reps, int n_reps, int64_t steps)
     const chromo_replica *R = &Rs;
     if (R->B != NB) return;
     if (R->counters[CHROMO_CNT_ERROR] != 0) return;
+    if (R->twist) {
+        // the speculative evaluator has no twist terms: a twisting chain runs
+        // with one wavefront (k_mc_run)
+        counters_error(R, ERR_TWIST_WAVES);
+        return;
+    }
     spec_run<NB, W>(R, steps);
 }
 

@@ -81,6 +81,7 @@ class EnergyModel:
         self.bonds = torch.as_tensor(np.ascontiguousarray(chain.bonds, dtype=np.float64), device=dev)
         m = L.ChromoEnergyModel()
         m.N, m.B = N, B
+        m.twist = 1 if getattr(chain, "twist", False) else 0
         m.t3, m.states, m.marks = self.t3.data_ptr(), self.states.data_ptr(), self.marks.data_ptr()
         m.bonds = self.bonds.data_ptr()
         m.nx, m.ny, m.nz = int(field.nx), int(field.ny), int(field.nz)
@@ -145,10 +146,13 @@ def chromo_energy(r: torch.Tensor, theta: torch.Tensor, model: EnergyModel) -> t
 # The reference's absolute energies on the GPU
 # ---------------------------------------------------------------------------
 
-def _reference_energies(poly, field):
+def _reference_energies(poly, field, twist: bool = True):
     from .fields import UniformDensityField
     from .mc.mc_sim import engine_for
     chain = poly.chain_spec()
+    if chain.twist and not twist:
+        chain.twist = False
+        chain.bonds = np.ascontiguousarray(np.asarray(chain.bonds)[:5])
     if isinstance(field, UniformDensityField):
         fs = field.field_spec()
         binders = [poly.beads[0].binders[b] for b in range(poly.num_binders)]
@@ -172,10 +176,12 @@ def _reference_energies(poly, field):
     return out.cpu().numpy(), dens
 
 
-def elastic_energy(poly) -> float:
-    """SSWLC.compute_E (polymers.pyx:1348-1381)."""
+def elastic_energy(poly, twist: bool = True) -> float:
+    """SSWLC.compute_E (polymers.pyx:1348-1381); for a twisting chain
+    SSTWLC.compute_E (polymers.pyx:2250-2286), or with twist=False
+    SSTWLC.compute_E_no_twist (polymers.pyx:2288-2320)."""
     from .fields import NullField
-    out, _ = _reference_energies(poly, NullField())
+    out, _ = _reference_energies(poly, NullField(), twist)
     return float(out[0])
 
 

@@ -32,12 +32,19 @@ class ChainSpec:
     t2: np.ndarray                # (N, 3)
     states: np.ndarray            # (N, B) int
     marks: np.ndarray             # (N, B) int
-    bonds: np.ndarray             # (5, N-1): eps_bend, eps_par, eps_perp, gamma, eta
+    bonds: np.ndarray             # (5, N-1): eps_bend, eps_par, eps_perp, gamma, eta;
+                                  # (7, N-1) for a twisting chain: + eps_twist, natural twist
     sites: Sequence[int]          # sites_per_bead per binder
     mu: Sequence[float]           # chemical_potential per binder
     F: np.ndarray                 # (B, n_mark_levels, sites+1) -log single-site Z
     max_binders: int = -1
     mu_adjust: float = 1.0
+    twist: bool = False           # SSTWLC (chromo/polymers.pyx:1889-2320)
+    bead_length: Optional[np.ndarray] = None     # (N-1,) bond lengths (host only)
+
+    @property
+    def bond_rows(self) -> int:
+        return 7 if self.twist else 5
 
     @property
     def N(self) -> int:
@@ -181,6 +188,8 @@ class DeviceReplicas:
                 raise NotImplementedError(f"binder count {c.B} outside 1..{L.MAX_BINDERS}")
             if c.N < 2:
                 raise ValueError("a chain needs at least 2 beads")
+            if np.asarray(c.bonds).shape != (c.bond_rows, c.N - 1):
+                raise ValueError(f"bonds must have shape ({c.bond_rows}, N-1) for this chain")
             if np.any(np.asarray(c.sites) > L.MAX_SITES):
                 raise NotImplementedError(f"sites_per_bead above {L.MAX_SITES}")
             if c.marks.size and (c.marks.min() < 0 or c.marks.max() >= L.MAX_MARK):
@@ -212,7 +221,8 @@ class DeviceReplicas:
 
         self.o_pos = offsets([9 * N for N in Ns])
         self.o_st = offsets([B * N for N, B in zip(Ns, Bs)])
-        self.o_bond = offsets([5 * (N - 1) for N in Ns])
+        self.o_bond = offsets([c.bond_rows * (N - 1) for c, N in zip(chains, Ns)])
+        self.twist = any(c.twist for c in chains)
         nbins = [f.n_bins if f.field_on else 1 for f in fields]
         self.nbins = nbins
         self.o_den = offsets([nb * (B + 1) for nb, B in zip(nbins, Bs)])
@@ -293,6 +303,7 @@ class DeviceReplicas:
             R.B = B
             R.n_ctrl = nctrl[i]
             R.rng_mode = self.rng_mode
+            R.twist = 1 if c.twist else 0
             R.field_on = 1 if f.field_on else 0
             if f.field_on:
                 (dx, dy, dz), nb, vol_bin, width, hw, hs = field_grid(f)
@@ -383,6 +394,8 @@ class DeviceReplicas:
         wavefronts per replica (default: ``self.waves``).  The result does not
         depend on ``waves``."""
         w = int(self.waves if waves is None else waves)
+        if self.twist:
+            w = 1       # the speculative evaluator has no twist terms
         L.check(self.lib.chromo_mc_run_tuned(self.tab_ptr, self.n, int(num_steps), w, self.binder_mask,
                                              _current_stream()), "chromo_mc_run_tuned")
         if sync:
@@ -391,7 +404,7 @@ class DeviceReplicas:
     def capture(self, num_steps: int = 1, *, waves: Optional[int] = None) -> "torch.cuda.CUDAGraph":
         """A HIP graph of ``run(num_steps)``: ``g.replay()`` advances every replica by
         num_steps sweeps with one graph launch (the launches record no memsets)."""
-        w = int(self.waves if waves is None else waves)
+        w = 1 if self.twist else int(self.waves if waves is None else waves)
         self.run(0, waves=w)             # sets the launch attributes outside the capture
         g = torch.cuda.CUDAGraph()
         s = torch.cuda.Stream(self.device)

@@ -381,6 +381,64 @@ class SSWLC(PolymerBase):
         return cls(name, r, t3=t3, t2=t2, bead_length=step_lengths, **kwargs)
 
 
+# polymers.pyx:38-40: base pair length (nm) and the bare natural twist per base pair
+LENGTH_BP = 0.332
+NATURAL_TWIST_BARE = 2 * np.pi / 10.5
+
+
+class SSTWLC(SSWLC):
+    """SSWLC with twist (polymers.pyx:1889-2320).
+
+    Each bond adds 0.5 eps_twist (omega - omega_0)^2, omega the twist angle
+    between the two beads' (t2, t3) frames (compute_twist_angle_omega,
+    polymers.pyx:3427-3461), omega_0 = bead_length 2 pi / (10.5 * 0.332 nm) the
+    natural twist of DNA, the difference wrapped into [-pi, pi);
+    eps_twist = lt / (delta lp).  Moves run with one wavefront per replica.
+    """
+
+    def __init__(self, name: str, r, *, bead_length, lp: float, lt: float, bead_rad: float = 5,
+                 t3=None, t2=None, states=None, binder_names=None, chemical_mods=None,
+                 chemical_mod_names=None, log_path: str = "", max_binders: int = -1, **kwargs):
+        self.lt = float(lt)
+        super().__init__(name, r, bead_length=bead_length, lp=lp, bead_rad=bead_rad, t3=t3, t2=t2,
+                         states=states, binder_names=binder_names, chemical_mods=chemical_mods,
+                         chemical_mod_names=chemical_mod_names, log_path=log_path,
+                         max_binders=max_binders)
+        self.required_attrs = np.array(["name", "r", "t3", "t2", "states", "binder_names",
+                                        "num_binders", "beads", "num_beads", "lp", "lt", "bead_rad"])
+
+    def _find_parameters(self, bead_length):
+        """SSTWLC._find_parameters (polymers.pyx:1956-2001): the dssWLC parameters
+        and eps_twist = lt / (delta lp) per bond."""
+        super()._find_parameters(bead_length)
+        self.eps_twist = self.lt / (np.asarray(self.delta) * self.lp)
+
+    def natural_twist(self) -> np.ndarray:
+        """bead_length * NATURAL_TWIST_BARE / LENGTH_BP per bond (polymers.pyx:2088-2090)."""
+        bl = np.asarray(self.bead_length, dtype=np.double).reshape(-1)
+        return bl * NATURAL_TWIST_BARE / LENGTH_BP
+
+    def bond_params(self) -> np.ndarray:
+        return np.vstack([super().bond_params(), self.eps_twist, self.natural_twist()])
+
+    def chain_spec(self) -> ChainSpec:
+        cs = super().chain_spec()
+        cs.twist = True
+        cs.bead_length = np.asarray(self.bead_length, dtype=np.double).reshape(-1).copy()
+        return cs
+
+    def compute_E(self) -> float:
+        """SSTWLC.compute_E (polymers.pyx:2250-2286), evaluated on the GPU."""
+        from .energy import elastic_energy
+        return elastic_energy(self)
+
+    def compute_E_no_twist(self) -> float:
+        """SSTWLC.compute_E_no_twist (polymers.pyx:2288-2320): the SSWLC energy of
+        the same chain, evaluated on the GPU."""
+        from .energy import elastic_energy
+        return elastic_energy(self, twist=False)
+
+
 class Chromatin(SSWLC):
     """SSWLC with the persistence length of DNA, lp = 53 nm (polymers.pyx:1821-1886)."""
 

@@ -26,7 +26,7 @@ from . import _lib as L
 from .binders import ReaderProtein
 from .engine import ChainSpec, CtrlSpec, FieldSpec, field_grid
 from .fields import field_prefactors
-from .polymers import binding_tables, sswlc_bond_params
+from .polymers import LENGTH_BP, NATURAL_TWIST_BARE, binding_tables, sswlc_bond_params
 
 BEAD_SPACING = 25.0
 LP_CHROMATIN = 53.0
@@ -159,6 +159,21 @@ def make_system(N: int, B: int, n_vox: int, *, seed: int, chi: float = 1.0, eps=
     _, _, vol_bin, _, _, _ = field_grid(fs)
     fs.pref, fs.intra, fs.cross = field_prefactors(binders, vol_bin)
     return System(chain, fs, default_controllers(N, per_cycle=per_cycle), binders)
+
+
+def make_twist_system(N: int, B: int, n_vox: int, *, seed: int, lt: float = 100.0, **kw) -> System:
+    """make_system with an SSTWLC chain (polymers.pyx:1889-2320): twist persistence
+    length lt (nm), eps_twist = lt / (delta lp) and the natural twist
+    bead_length 2 pi / (10.5 * 0.332) on every bond."""
+    s = make_system(N, B, n_vox, seed=seed, **kw)
+    bl = np.full(N - 1, BEAD_SPACING)
+    delta = sswlc_bond_params(bl, LP_CHROMATIN)[0]
+    et = lt / (delta * LP_CHROMATIN)
+    tw0 = bl * NATURAL_TWIST_BARE / LENGTH_BP
+    s.chain.bonds = np.vstack([s.chain.bonds, et, tw0])
+    s.chain.twist = True
+    s.chain.bead_length = bl
+    return s
 
 
 def config_plain_cpu(seed: int = 1) -> System:

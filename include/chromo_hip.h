@@ -110,7 +110,9 @@ enum chromo_replica_error {
     CHROMO_ERR_NBINDERS = 4,    /* binder count outside 1..4 */
     CHROMO_ERR_SPIN = 5,        /* a rejection or probe loop passed its bound (amplitude below one bead:
                                    the reference loops for ever) */
-    CHROMO_ERR_ARENA = 6        /* a scratch-arena index check failed */
+    CHROMO_ERR_ARENA = 6,       /* a scratch-arena index check failed */
+    CHROMO_ERR_TWIST_WAVES = 7  /* a twisting chain (SSTWLC) launched with more than one wavefront
+                                   per replica: it runs with one */
 };
 
 /* Everything the kernels need about one replica (one chain in one field).  The
@@ -160,6 +162,9 @@ typedef struct {
     int64_t n_sub[3];           /* sub-bins per axis: n_points * n (n_points even) */
     double sub_width[3];        /* voxel side / n_points */
     double half_points;         /* n_points / 2 */
+    int32_t twist;              /* 1: SSTWLC (chromo/polymers.pyx:1889-2320): bonds has 7 rows, the
+                                   last two eps_twist and the natural twist angle of each bond */
+    int32_t pad_tw;
 } chromo_replica;
 
 /* ---- library ---------------------------------------------------------- */
@@ -259,11 +264,13 @@ int chromo_capped_exponential_tape(const int32_t *d_draws, const int64_t *d_wind
 /* ---- absolute energies and adjoint -------------------------------------- */
 
 /* One chain in one field, for the energy entry points.  Device pointers:
- * t3 (N x 3), states / marks (B x N), bonds (5 x (N-1)). */
+ * t3 (N x 3), states / marks (B x N), bonds (5 x (N-1); 7 x (N-1) when twist). */
 typedef struct {
     int64_t N;
     int32_t B;
-    int32_t pad_;
+    int32_t twist;      /* 1: SSTWLC bonds; chromo_energy_reference adds the twist term of
+                           SSTWLC.compute_E (polymers.pyx:2250-2286), the adjoint entry
+                           points reject it */
     const double *t3;
     const int8_t *states;
     const int8_t *marks;

@@ -345,6 +345,8 @@ struct or_sim {
     double *r, *t3, *t2, *r_trial, *t3_trial, *t2_trial;
     int64_t *states, *states_trial, *marks;
     double *eps_bend, *eps_par, *eps_perp, *gamma, *eta;
+    int twist;
+    double *eps_twist, *bead_length;
     int64_t max_binders;
     int64_t sites[OR_MAXB];
     double mu[OR_MAXB];
@@ -435,6 +437,13 @@ or_sim *or_new(const or_spec *sp, int32_t rng_mode, uint32_t seed, uint32_t stre
         memcpy(s->eps_perp, sp->eps_perp, (size_t)(N - 1) * sizeof(double));
         memcpy(s->gamma, sp->gamma, (size_t)(N - 1) * sizeof(double));
         memcpy(s->eta, sp->eta, (size_t)(N - 1) * sizeof(double));
+    }
+    s->twist = sp->twist != 0;
+    s->eps_twist = (double *)calloc(1, nb);
+    s->bead_length = (double *)calloc(1, nb);
+    if (s->twist && N > 1) {
+        memcpy(s->eps_twist, sp->eps_twist, (size_t)(N - 1) * sizeof(double));
+        memcpy(s->bead_length, sp->bead_length, (size_t)(N - 1) * sizeof(double));
     }
     s->max_binders = sp->max_binders;
     memcpy(s->sites, sp->sites, sizeof(s->sites));
@@ -531,6 +540,8 @@ void or_free(or_sim *s)
     free(s->eps_perp);
     free(s->gamma);
     free(s->eta);
+    free(s->eps_twist);
+    free(s->bead_length);
     free(s->wrap);
     free(s->density);
     free(s->access_vols);
@@ -961,6 +972,98 @@ static inline double E_pair(const or_sim *s, const double *bend, double dr_par, 
            0.5 * s->eps_perp[b] * dot3(dr_perp, dr_perp);
 }
 
+/* ---- twist: SSTWLC, polymers.pyx:1889-2320 ---- */
+
+/* polymers.pyx:38-40 */
+static const double LENGTH_BP = 0.332;
+static const double NATURAL_TWIST_BARE = 2 * M_PI / 10.5;
+
+/* compute_twist_angle_omega, polymers.pyx:3427-3461 (np.dot of 3-vectors taken
+ * as the left-to-right sum; np.arctan2 is glibc atan2) */
+double or_twist_omega(const double *t2_0, const double *t3_0, const double *t2_1, const double *t3_1)
+{
+    double t1_0[3] = {t2_0[1] * t3_0[2] - t2_0[2] * t3_0[1], t2_0[2] * t3_0[0] - t2_0[0] * t3_0[2],
+                      t2_0[0] * t3_0[1] - t2_0[1] * t3_0[0]};
+    double t1_1[3] = {t2_1[1] * t3_1[2] - t2_1[2] * t3_1[1], t2_1[2] * t3_1[0] - t2_1[0] * t3_1[2],
+                      t2_1[0] * t3_1[1] - t2_1[1] * t3_1[0]};
+    return atan2(dot3(t2_0, t1_1) - dot3(t1_0, t2_1), dot3(t1_0, t1_1) + dot3(t2_0, t2_1));
+}
+
+/* Python float a // b (Objects/floatobject.c float_floor_div): E_pair_with_twist
+ * evaluates its wrap with np.pi operands, i.e. on Python floats. */
+static double py_floor_div(double vx, double wx)
+{
+    double mod = fmod(vx, wx);
+    double div = (vx - mod) / wx;
+    if (mod) {
+        if ((wx < 0) != (mod < 0)) div -= 1.0;
+    }
+    double floordiv;
+    if (div) {
+        floordiv = floor(div);
+        if (div - floordiv > 0.5) floordiv += 1.0;
+    } else {
+        floordiv = copysign(0.0, vx / wx);
+    }
+    return floordiv;
+}
+
+/* E_pair_with_twist, polymers.pyx:2050-2102 */
+static inline double E_pair_tw(const or_sim *s, const double *bend, double dr_par, const double *dr_perp,
+                               double omega, int64_t b)
+{
+    double d = omega - (s->bead_length[b] * NATURAL_TWIST_BARE / LENGTH_BP);
+    d -= 2 * M_PI * py_floor_div(d + M_PI, 2 * M_PI);
+    return 0.5 * s->eps_bend[b] * dot3(bend, bend) +
+           0.5 * s->eps_par[b] * pow(dr_par - s->gamma[b], 2.0) +
+           0.5 * s->eps_perp[b] * dot3(dr_perp, dr_perp) +
+           0.5 * s->eps_twist[b] * pow(d, 2.0);
+}
+
+/* bead_pair_dE_poly_forward_with_twist, polymers.pyx:2104-2175 */
+static double pair_dE_forward_tw(const or_sim *s, const double *r0, const double *r1, const double *tr1,
+                                 const double *t30, const double *t31, const double *tt31,
+                                 const double *t20, const double *t21, const double *tt21, int64_t b)
+{
+    double omega = or_twist_omega(t20, t30, t21, t31);
+    double omega_t = or_twist_omega(t20, t30, tt21, tt31);
+    double dr_t[3], dr[3], drp_t[3], drp[3], bend_t[3], bend[3];
+    for (int i = 0; i < 3; ++i) {
+        dr_t[i] = tr1[i] - r0[i];
+        dr[i] = r1[i] - r0[i];
+    }
+    double par_t = dot3(t30, dr_t), par = dot3(t30, dr);
+    for (int i = 0; i < 3; ++i) {
+        drp_t[i] = dr_t[i] - t30[i] * par_t;
+        drp[i] = dr[i] - t30[i] * par;
+        bend_t[i] = tt31[i] - t30[i] - drp_t[i] * s->eta[b];
+        bend[i] = t31[i] - t30[i] - drp[i] * s->eta[b];
+    }
+    return E_pair_tw(s, bend_t, par_t, drp_t, omega_t, b) - E_pair_tw(s, bend, par, drp, omega, b);
+}
+
+/* bead_pair_dE_poly_reverse_with_twist, polymers.pyx:2177-2248 */
+static double pair_dE_reverse_tw(const or_sim *s, const double *r0, const double *tr0, const double *r1,
+                                 const double *t30, const double *tt30, const double *t31,
+                                 const double *t20, const double *tt20, const double *t21, int64_t b)
+{
+    double omega = or_twist_omega(t20, t30, t21, t31);
+    double omega_t = or_twist_omega(tt20, tt30, t21, t31);
+    double dr_t[3], dr[3], drp_t[3], drp[3], bend_t[3], bend[3];
+    for (int i = 0; i < 3; ++i) {
+        dr_t[i] = r1[i] - tr0[i];
+        dr[i] = r1[i] - r0[i];
+    }
+    double par_t = dot3(tt30, dr_t), par = dot3(t30, dr);
+    for (int i = 0; i < 3; ++i) {
+        drp_t[i] = dr_t[i] - tt30[i] * par_t;
+        drp[i] = dr[i] - t30[i] * par;
+        bend_t[i] = t31[i] - tt30[i] - drp_t[i] * s->eta[b];
+        bend[i] = t31[i] - t30[i] - drp[i] * s->eta[b];
+    }
+    return E_pair_tw(s, bend_t, par_t, drp_t, omega_t, b) - E_pair_tw(s, bend, par, drp, omega, b);
+}
+
 /* bead_pair_dE_poly_forward, polymers.pyx:1177-1277 */
 static double pair_dE_forward(const or_sim *s, const double *r0, const double *r1, const double *tr1,
                               const double *t30, const double *t31, const double *tt31, int64_t b)
@@ -999,10 +1102,25 @@ static double pair_dE_reverse(const or_sim *s, const double *r0, const double *t
     return E_pair(s, bend_t, par_t, drp_t, b) - E_pair(s, bend, par, drp, b);
 }
 
-/* continuous_dE_poly, polymers.pyx:1084-1146 */
+/* continuous_dE_poly, polymers.pyx:1084-1146 (SSTWLC: polymers.pyx:2003-2044) */
 static double continuous_dE(const or_sim *s, int64_t ind0, int64_t indf)
 {
     double dE = 0;
+    if (s->twist) {
+        if (ind0 != 0) {
+            int64_t a = ind0 - 1;
+            dE += pair_dE_forward_tw(s, s->r + 3 * a, s->r + 3 * ind0, s->r_trial + 3 * ind0,
+                                     s->t3 + 3 * a, s->t3 + 3 * ind0, s->t3_trial + 3 * ind0,
+                                     s->t2 + 3 * a, s->t2 + 3 * ind0, s->t2_trial + 3 * ind0, a);
+        }
+        if (indf != s->N) {
+            int64_t a = indf - 1;
+            dE += pair_dE_reverse_tw(s, s->r + 3 * a, s->r_trial + 3 * a, s->r + 3 * indf,
+                                     s->t3 + 3 * a, s->t3_trial + 3 * a, s->t3 + 3 * indf,
+                                     s->t2 + 3 * a, s->t2_trial + 3 * a, s->t2 + 3 * indf, a);
+        }
+        return dE;
+    }
     if (ind0 != 0) {
         int64_t a = ind0 - 1;
         dE += pair_dE_forward(s, s->r + 3 * a, s->r + 3 * ind0, s->r_trial + 3 * ind0, s->t3 + 3 * a,
@@ -1558,18 +1676,20 @@ double or_field_dE_tape(or_sim *s, const int64_t *inds, int64_t n, const double 
 }
 
 double or_elastic_dE_tape(or_sim *s, int32_t move, const int64_t *inds, int64_t n,
-                          const double *r_trial, const double *t3_trial)
+                          const double *r_trial, const double *t3_trial, const double *t2_trial)
 {
     for (int64_t i = 0; i < n; ++i)
         for (int j = 0; j < 3; ++j) {
             s->r_trial[3 * inds[i] + j] = r_trial[3 * i + j];
             s->t3_trial[3 * inds[i] + j] = t3_trial[3 * i + j];
+            if (t2_trial) s->t2_trial[3 * inds[i] + j] = t2_trial[3 * i + j];
         }
     double dE = poly_dE(s, move, inds, n);
     for (int64_t i = 0; i < n; ++i)
         for (int j = 0; j < 3; ++j) {
             s->r_trial[3 * inds[i] + j] = s->r[3 * inds[i] + j];
             s->t3_trial[3 * inds[i] + j] = s->t3[3 * inds[i] + j];
+            s->t2_trial[3 * inds[i] + j] = s->t2[3 * inds[i] + j];
         }
     return dE;
 }
@@ -1582,9 +1702,32 @@ double or_binding_dE_tape(or_sim *s, int64_t ind0, int64_t indf, const int64_t *
     return dE;
 }
 
+/* SSTWLC.compute_E, polymers.pyx:2250-2286 */
+static double elastic_E_tw(const or_sim *s)
+{
+    double E = 0;
+    for (int64_t i = 1; i < s->N; ++i) {
+        int64_t a = i - 1;
+        const double *r0 = s->r + 3 * a, *r1 = s->r + 3 * i, *t30 = s->t3 + 3 * a,
+                     *t31 = s->t3 + 3 * i, *t20 = s->t2 + 3 * a, *t21 = s->t2 + 3 * i;
+        double omega = or_twist_omega(t20, t30, t21, t31);
+        double dr[3], drp[3], bend[3];
+        for (int j = 0; j < 3; ++j) dr[j] = r1[j] - r0[j];
+        double par = dot3(t30, dr);
+        for (int j = 0; j < 3; ++j) drp[j] = dr[j] - t30[j] * par;
+        for (int j = 0; j < 3; ++j) {
+            bend[j] = t31[j];
+            bend[j] += -t30[j] - s->eta[a] * drp[j];
+        }
+        E += E_pair_tw(s, bend, par, drp, omega, a);
+    }
+    return E;
+}
+
 /* SSWLC.compute_E, polymers.pyx:1348-1381 */
 double or_elastic_E(const or_sim *s)
 {
+    if (s->twist) return elastic_E_tw(s);
     double E = 0;
     for (int64_t i = 1; i < s->N; ++i) {
         int64_t a = i - 1;

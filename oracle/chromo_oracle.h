@@ -47,6 +47,10 @@ typedef struct {
     const double *access_vols;          /* n_bins accessible volumes (copied), or NULL: vol_bin */
     int32_t fast_field;                 /* 1: sub-bin lookup weights in the move dE */
     int64_t n_points;                   /* sub-bins per voxel side (made even) */
+    /* SSTWLC (polymers.pyx:1889-2320): twist modulus per bond and the bead lengths
+     * the natural twist is taken from; twist = 0 for SSWLC */
+    int32_t twist;
+    const double *eps_twist, *bead_length;      /* N-1 each */
 } or_spec;
 
 typedef struct {
@@ -99,9 +103,12 @@ double or_field_dE_tape(or_sim *s, const int64_t *inds, int64_t n, const double 
                         const int64_t *states_trial, int32_t state_change,
                         double *delta_rho, int64_t *bins, int64_t *n_bins_out);
 double or_elastic_dE_tape(or_sim *s, int32_t move, const int64_t *inds, int64_t n,
-                          const double *r_trial, const double *t3_trial);
+                          const double *r_trial, const double *t3_trial, const double *t2_trial);
+/* compute_twist_angle_omega (polymers.pyx:3427-3461) */
+double or_twist_omega(const double *t2_0, const double *t3_0, const double *t2_1, const double *t3_1);
 double or_binding_dE_tape(or_sim *s, int64_t ind0, int64_t indf, const int64_t *states_trial);
-/* Absolute energies: SSWLC.compute_E and UniformDensityField.compute_E (as in the reference). */
+/* Absolute energies: SSWLC.compute_E (SSTWLC.compute_E for a twisting chain) and
+ * UniformDensityField.compute_E (as in the reference). */
 double or_elastic_E(const or_sim *s);
 double or_field_E(or_sim *s);
 

@@ -42,6 +42,7 @@ class OrSpec(ctypes.Structure):
         ("confine_type", ctypes.c_int32), ("confine_length", ctypes.c_double),
         ("access_vols", _dp),
         ("fast_field", ctypes.c_int32), ("n_points", ctypes.c_int64),
+        ("twist", ctypes.c_int32), ("eps_twist", _dp), ("bead_length", _dp),
     ]
 
 
@@ -100,7 +101,9 @@ def lib():
         h.or_field_dE_tape.restype = ctypes.c_double
         h.or_field_dE_tape.argtypes = [V, _ip, ctypes.c_int64, _dp, _ip, ctypes.c_int32, _dp, _ip, _ip]
         h.or_elastic_dE_tape.restype = ctypes.c_double
-        h.or_elastic_dE_tape.argtypes = [V, ctypes.c_int32, _ip, ctypes.c_int64, _dp, _dp]
+        h.or_elastic_dE_tape.argtypes = [V, ctypes.c_int32, _ip, ctypes.c_int64, _dp, _dp, _dp]
+        h.or_twist_omega.restype = ctypes.c_double
+        h.or_twist_omega.argtypes = [_dp, _dp, _dp, _dp]
         h.or_binding_dE_tape.restype = ctypes.c_double
         h.or_binding_dE_tape.argtypes = [V, ctypes.c_int64, ctypes.c_int64, _ip]
         h.or_elastic_E.restype = ctypes.c_double
@@ -177,6 +180,15 @@ class OracleSim:
         for a in rows:
             keep(a)
         sp.eps_bend, sp.eps_par, sp.eps_perp, sp.gamma, sp.eta = [_dptr(a) for a in rows]
+        if getattr(chain, "twist", False):
+            # eps_twist from the chain's bond table, the natural twist from the bead
+            # lengths (the oracle recomputes it as E_pair_with_twist does)
+            et = np.ascontiguousarray(bonds[5])
+            bl = _d(chain.bead_length)
+            keep(et)
+            keep(bl)
+            sp.twist = 1
+            sp.eps_twist, sp.bead_length = _dptr(et), _dptr(bl)
         sp.max_binders = int(chain.max_binders)
         F = np.asarray(chain.F, dtype=np.float64)
         sp.n_mark_levels = F.shape[1]
@@ -258,10 +270,11 @@ class OracleSim:
                                      int(state_change), _dptr(delta), _iptr(bins), ctypes.byref(k))
         return dE, bins[:k.value].copy(), delta
 
-    def elastic_dE_tape(self, move, inds, r_trial, t3_trial):
+    def elastic_dE_tape(self, move, inds, r_trial, t3_trial, t2_trial=None):
         inds = _i(inds)
+        t2 = None if t2_trial is None else _dptr(_d(t2_trial))
         return self.h.or_elastic_dE_tape(self.p, int(move), _iptr(inds), len(inds), _dptr(_d(r_trial)),
-                                         _dptr(_d(t3_trial)))
+                                         _dptr(_d(t3_trial)), t2)
 
     def binding_dE_tape(self, ind0, indf, states_trial):
         return self.h.or_binding_dE_tape(self.p, int(ind0), int(indf), _iptr(_i(states_trial)))
@@ -282,6 +295,12 @@ def cexp_trunc(draws, windows):
     out = np.zeros(len(d), dtype=np.int64)
     lib().or_cexp_trunc(d.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _iptr(w), len(d), _iptr(out))
     return out
+
+
+def twist_omega(t2_0, t3_0, t2_1, t3_1):
+    """compute_twist_angle_omega (chromo/polymers.pyx:3427-3461)."""
+    a = [_d(v) for v in (t2_0, t3_0, t2_1, t3_1)]
+    return lib().or_twist_omega(*[_dptr(v) for v in a])
 
 
 def py_round2(x):

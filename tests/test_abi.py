@@ -64,4 +64,4 @@ def test_replica_error_codes_agree():
     dev = (ROOT / "chromo_amd" / "csrc" / "chromo_device.hpp").read_text()
     device = {int(v): n for n, v in re.findall(r"\bERR_(\w+)\s*=\s*(\d+)", dev)}
     assert header == device
-    assert set(header) == set(_lib.ERROR_TEXT) == set(range(1, 7))
+    assert set(header) == set(_lib.ERROR_TEXT) == set(range(1, 8))

@@ -9,7 +9,10 @@ kernel branches the other parity tests do not reach.
   stream: every decision, window and voxel count equal over a long horizon;
 * voxels over vf_limit (the 1e99 volume-fraction penalty), spherical and
   cubical confinement, max_binders (the 1e25 binder penalty) and
-  mu_adjust_factor != 1 with chemical potentials of both signs.
+  mu_adjust_factor != 1 with chemical potentials of both signs;
+* a twisting chain (SSTWLC) on every evaluator (one-wavefront segments,
+  64-bead segments, chunked large moves, the plain path) and its absolute
+  energy.
 
 Bar as in test_gpu_parity.py: windows, voxel counts, decisions, Metropolis
 draws and binder states equal; dE within 1e-9 kT absolute or 1e-12 relative
@@ -304,3 +307,40 @@ def test_fast_field(oracle, npts, waves):
         assert np.array_equal(delta_g, delta_o[bins_g])
         assert _dE_close(np.array([dEg]), np.array([dEo]))[0]
     _run_pair(oracle, s, 8, waves=waves)
+
+
+# ------------------------------------------------------------------ twist ---
+
+@pytest.mark.parametrize("N,launch", [(1500, "run"), (300, "run"), (1500, "plain")])
+def test_twist_chain(oracle, N, launch):
+    """SSTWLC: E_pair_with_twist in the forward and reverse pairs of every move
+    (chromo/polymers.pyx:2003-2248).  The speculative evaluator has no twist
+    terms, so run(waves=2) runs one wavefront and gives the same chain."""
+    say("twist", N, launch)
+    s = W.make_twist_system(N, 2, 12, seed=71 + N, lt=100.0, eps=[-4, -2], mu=[-0.4, 0.3])
+    _run_pair(oracle, s, 10, waves=2, launch=launch)
+
+
+def test_twist_absolute_energy(oracle):
+    """SSTWLC.compute_E and compute_E_no_twist on the GPU against the oracle; the
+    energy adjoint refuses a twisting chain."""
+    import torch
+    from chromo_amd.energy import EnergyModel, energy
+    from chromo_amd.engine import FieldSpec
+    from chromo_amd.polymers import SSTWLC
+    say("twist energy")
+    s = W.make_twist_system(2000, 2, 12, seed=5, lt=100.0)
+    c = s.chain
+    p = SSTWLC("tw", c.r, t3=c.t3, t2=c.t2, bead_length=c.bead_length, lp=W.LP_CHROMATIN, lt=100.0)
+    assert np.array_equal(p.bond_params(), c.bonds)
+    null = FieldSpec(field_on=False)
+    Eo = oracle.OracleSim(c, null).elastic_E()
+    c0 = copy.deepcopy(c)
+    c0.twist, c0.bonds = False, c0.bonds[:5]
+    Eo0 = oracle.OracleSim(c0, null).elastic_E()
+    assert abs(p.compute_E() - Eo) <= 1e-12 * abs(Eo)
+    assert abs(p.compute_E_no_twist() - Eo0) <= 1e-12 * abs(Eo0)
+    assert Eo > Eo0
+    m = EnergyModel(c, s.field, s.binders)
+    with pytest.raises(L.ChromoError, match="twist"):
+        energy(m, torch.as_tensor(c.r), torch.as_tensor(m.theta0))

@@ -1,7 +1,9 @@
 """Closed forms and known answers for the oracle functions that the reference's
 own tests leave open: the SSWLC bond energy E_pair, Python's round(x, 2), the
 binding dE's chemical-potential branches under mu_adjust_factor != 1 and its
-max_binders penalty, and the confinement energy of both confinement types.
+max_binders penalty, the confinement energy of both confinement types, and the
+SSTWLC twist term (twist angle, natural twist, wrap, which frames a move's
+forward and reverse pairs take as trial).
 
 Each check is built so that a plausible restatement mistake fails it: a swapped
 eps_par / eps_perp, a dropped eta or gamma, a projection on the wrong tangent,
@@ -9,6 +11,7 @@ a missing 1/2, the mu > 0 branch using f instead of 2 - f, a penalty of the
 wrong sign or size, or the cubical confinement counting the current
 configuration (chromo/fields.pyx:188-193 never does).
 """
+import copy
 import math
 
 import numpy as np
@@ -396,3 +399,108 @@ def test_fast_field_weights_are_sub_bin_edge_weights(oracle, npts):
     # the weights differ from the exact ones by less than one sub-bin per axis
     d_exact_new, _ = _density_bruteforce(s.field, rt, st[inds])
     assert np.max(np.abs(d_new - d_exact_new)) * vb <= 7 * 3.0 / npts + 1e-12
+
+
+# ---------------------------------------------------------------- twist ---
+# SSTWLC (chromo/polymers.pyx:1889-2320): E_pair + eps_twist/2 wrap(omega - omega_0)^2,
+# wrap into [-pi, pi), omega_0 = bead_length 2 pi / (10.5 * 0.332).
+
+def _rot_about(axis, v, ang):
+    """Rodrigues rotation of v about the unit axis by ang (right hand)."""
+    axis = np.asarray(axis, dtype=float)
+    return (v * np.cos(ang) + np.cross(axis, v) * np.sin(ang)
+            + axis * (axis @ v) * (1 - np.cos(ang)))
+
+
+def test_twist_omega_known_answers(oracle):
+    """tests/test_linker_lengths.py:116-157 of the reference: straight chain in x
+    with t2 = y: omega 0; t2_1 = -y: pi; t2_1 = z: pi/2.  Then frames turned by
+    phi about an arbitrary t3 give +phi, and the swapped pair -phi."""
+    x, y, z = np.eye(3)
+    assert abs(oracle.twist_omega(y, x, y, x)) < 1e-15
+    assert math.isclose(abs(oracle.twist_omega(y, x, -y, x)), math.pi, rel_tol=1e-15)
+    assert math.isclose(oracle.twist_omega(y, x, z, x), math.pi / 2, rel_tol=1e-15)
+    rs = np.random.RandomState(11)
+    for _ in range(200):
+        t3 = rs.standard_normal(3)
+        t3 /= np.linalg.norm(t3)
+        t2 = np.cross(t3, rs.standard_normal(3))
+        t2 /= np.linalg.norm(t2)
+        phi = rs.uniform(-3.1, 3.1)
+        t2b = _rot_about(t3, t2, phi)
+        assert math.isclose(oracle.twist_omega(t2, t3, t2b, t3), phi, rel_tol=0, abs_tol=1e-12)
+        assert math.isclose(oracle.twist_omega(t2b, t3, t2, t3), -phi, rel_tol=0, abs_tol=1e-12)
+
+
+def _twisted_line(N, bl, turn):
+    """Straight chain along x (bond lengths bl), t3 = x, frame n turned by turn[:n].sum()."""
+    x, y = np.eye(3)[0], np.eye(3)[1]
+    r = np.vstack([np.zeros(3), np.cumsum(bl[:, None] * x, axis=0)])
+    t3 = np.tile(x, (N, 1))
+    ang = np.concatenate([[0.0], np.cumsum(turn)])
+    t2 = np.stack([_rot_about(x, y, a) for a in ang])
+    return r, t3, t2
+
+
+def _twist_chain(r, t3, t2, bl, lp, lt):
+    from chromo_amd.polymers import SSTWLC
+    p = SSTWLC("tw", r, t3=t3, t2=t2, bead_length=bl, lp=lp, lt=lt)
+    c = p.chain_spec()
+    return p, c
+
+
+@pytest.mark.parametrize("bp", [36, 10.5, 17.0, 31.5 + 2])
+def test_twist_energy_closed_forms(oracle, bp):
+    """Frames turned by omega_0 + phi_n per bond: the twist energy is
+    sum eps_twist/2 wrap(phi_n)^2 on top of the untwisted SSWLC energy -- with
+    omega_0 above pi (36 bp is 3.4 turns) the wrap is what keeps it small; 10.5 bp
+    is a whole turn (omega_0 = 2 pi, omega = 0)."""
+    N = 9
+    lp, lt = 50.0, 100.0
+    bl = np.full(N - 1, bp * 0.332) * (1 + 0.01 * np.arange(N - 1))
+    w0 = bl * (2 * np.pi / 10.5) / 0.332
+    phi = np.linspace(-1.2, 1.4, N - 1)
+    r, t3, t2 = _twisted_line(N, bl, w0 + phi)
+    p, c = _twist_chain(r, t3, t2, bl, lp, lt)
+    et = lt / ((bl / lp) * lp)
+    assert np.allclose(p.eps_twist, et, rtol=1e-15)
+    E = oracle.OracleSim(c, NULL_FIELD).elastic_E()
+    c0 = copy.deepcopy(c)
+    c0.twist = False
+    c0.bonds = c0.bonds[:5]
+    E0 = oracle.OracleSim(c0, NULL_FIELD).elastic_E()
+    exp = float(np.sum(0.5 * et * phi ** 2))
+    assert math.isclose(E - E0, exp, rel_tol=1e-9)
+    # a whole extra turn on one bond changes nothing
+    turn = w0 + phi
+    turn[3] += 2 * np.pi
+    r, t3, t2 = _twisted_line(N, bl, turn)
+    E2 = oracle.OracleSim(_twist_chain(r, t3, t2, bl, lp, lt)[1], NULL_FIELD).elastic_E()
+    assert math.isclose(E2, E, rel_tol=1e-9)
+
+
+def test_twist_dE_of_one_bead_turn(oracle):
+    """A tangent rotation of middle bead k about its own t3 by a twists bond k-1
+    by +a and bond k by -a; on a chain at natural twist its dE is
+    eps_twist_{k-1} a^2 / 2 + eps_twist_k a^2 / 2 (the bend and stretch terms do
+    not change).  A move that kept the reverse pair's current frame, or turned the
+    forward pair's fixed bead, gives another value."""
+    N = 8
+    lp, lt = 50.0, 100.0
+    bl = np.full(N - 1, 12.0) * (1 + 0.05 * np.arange(N - 1))
+    w0 = bl * (2 * np.pi / 10.5) / 0.332
+    r, t3, t2 = _twisted_line(N, bl, w0)
+    _, c = _twist_chain(r, t3, t2, bl, lp, lt)
+    sim = oracle.OracleSim(c, NULL_FIELD)
+    et = lt / ((bl / lp) * lp)
+    k, a = 4, 0.7
+    t2k = _rot_about(t3[k], t2[k], a)
+    dE = sim.elastic_dE_tape(3, np.array([k]), r[[k]], t3[[k]], t2k[None])
+    exp = 0.5 * et[k - 1] * a * a + 0.5 * et[k] * a * a
+    assert math.isclose(dE, exp, rel_tol=1e-9)
+    # the untwisted model sees no change
+    c0 = copy.deepcopy(c)
+    c0.twist = False
+    c0.bonds = c0.bonds[:5]
+    sim0 = oracle.OracleSim(c0, NULL_FIELD)
+    assert abs(sim0.elastic_dE_tape(3, np.array([k]), r[[k]], t3[[k]], t2k[None])) < 1e-12
