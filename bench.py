@@ -6,10 +6,13 @@ the persistent kernel, chain lengths N ~ U{2000..50000}, 32^3..64^3 voxels of
 marks, Philox streams.  One *step* is one mc_sim sweep of every replica
 (all_moves: 161 proposals per replica).
 
-The chains are first run ``--equilibrate`` sweeps (default 400) so that the
-move controllers reach their steady-state amplitudes (the mean move grows from
-~2.6 beads at the start to ~6.8); the timed window is the sweep range reported
-in ``config.sweeps_timed``.  After the warmup, chromo_amd.autotune times every
+The chains are first run ``--equilibrate`` sweeps (default 2,500) so that the
+move controllers reach their steady-state amplitudes: the controllers raise the
+crank-shaft, slide and end-pivot windows by one bead every ~19 sweeps until they
+reach their 150-bead bound (~2,200 sweeps), and the mean move grows from ~3.7
+beads over the first 200 sweeps to ~20.6.  The timed window is the sweep range
+reported in ``config.sweeps_timed``; a sweep there costs ~1.8x one of the first
+400 sweeps, so the first sweeps would overstate the rate by that much.  After the warmup, chromo_amd.autotune times every
 pair (wavefronts per replica, sweeps per graph launch) and keeps the fastest;
 all settings produce the same chains.  The timed steps replay a captured HIP
 graph of that many sweeps (``--no-graph``: direct launches).
@@ -86,7 +89,7 @@ def bench_batched(args):
     # steady state: the controllers raise the move amplitudes over the first few
     # hundred sweeps; run them as part of preparing the workload
     while done < args.equilibrate:
-        k = min(100, args.equilibrate - done)
+        k = min(500, args.equilibrate - done)
         dr.run(k, sync=False, waves=2)
         done += k
     dr.raise_on_error()
@@ -304,8 +307,10 @@ def main():
     ap.add_argument("--sweeps-per-launch", default="auto",
                     help="sweeps per graph launch, or 'auto' (with --waves auto: time the divisors of "
                          "--steps among 1, 2, 4, 5, 10, 20, ...)")
-    ap.add_argument("--equilibrate", type=int, default=400,
-                    help="sweeps run before the warmup so that the move amplitudes are at steady state")
+    ap.add_argument("--equilibrate", type=int, default=2500,
+                    help="sweeps run before the warmup so that the move amplitudes are at steady state "
+                         "(crank-shaft, slide and end-pivot windows reach their 150-bead bound after "
+                         "~2,200 sweeps; the mean move then holds ~20.6 beads)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 256 MiB device to device 5 times (calibrates rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE counters against a known byte count)")
@@ -357,10 +362,10 @@ def main():
     prof = ROOT / "profiles" / f"traffic_{kname}.json"
     if prof.exists():
         try:
+            # measured on this workload's steady-state timed window, per sweep; a
+            # launch of S sweeps moves S times that
             tj = json.loads(prof.read_text())
-            # measured on this workload's timed window at this many sweeps per launch
-            if int(tj.get("sweeps_per_launch", 1)) == b["spl"]:
-                traffic = tj.get("hbm_bytes_per_launch")
+            traffic = float(tj["hbm_bytes_per_sweep"]) * b["spl"]
         except Exception:
             traffic = None
     out = {
