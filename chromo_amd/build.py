@@ -7,7 +7,7 @@ chromo_amd/libchromo_hip.so:
 
 * chromo_hip.hip: one-wavefront kernels, RNG, density, tape, host entry points;
 * chromo_energy.hip: energies and their adjoint;
-* chromo_spec.hip, once per W in (2, 4, 8): the speculative kernel with W
+* chromo_spec.hip, once per W in SPEC_WAVES (2): the speculative kernel with W
   wavefronts per replica.
 
 -ffp-contract=off keeps every multiply and add separately rounded, as in the

@@ -57,6 +57,7 @@ struct SpecCtl {
     int32_t tag;                              // the round's number in the launch (never repeats)
     int32_t ready;                            // = tag once the later slots' conflict masks are set
     int32_t ins;                              // = tag once slot 0's voxels are in the conflict table
+    int32_t stall;                            // = tag when a later wavefront gave up waiting for ins
 };
 
 // LDS exchange area of the one-at-a-time path (seq_step).
@@ -851,8 +852,14 @@ __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng
                     if (++spins >= SPIN_LIMIT) break;
                 }
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                // a table that did not arrive: the slot waits for the next round
-                if (spins >= SPIN_LIMIT && mine && lane == 0) L.res[wv].overflow = 1;
+                // a table that did not arrive: the slot waits for the next round, and
+                // since this wavefront may then read slot 0's result before it is
+                // written, wavefront 0 stops the replica (ERR_SPIN) rather than let
+                // the two resolve the round differently
+                if (spins >= SPIN_LIMIT && lane == 0) {
+                    if (mine) L.res[wv].overflow = 1;
+                    L.ctl->stall = tag;
+                }
                 wave_sync();
                 if (mine && !L.res[wv].resample && !L.res[wv].overflow)
                     spec_conflicts<RING>(ring, h, L.res, L, L.mine, wv);
@@ -872,6 +879,7 @@ __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng
                         }
                     }
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    if (uni_i32(L.ctl->stall) == tag) pend = ERR_SPIN;
                 }
             }
             // resolution, identical in every wavefront; lane s reads slot s's result,
@@ -965,6 +973,7 @@ __device__ __forceinline__ void spec_run(const chromo_replica *R, int64_t steps)
             L.ctl->tag = 0;
             L.ctl->ready = -1;
             L.ctl->ins = -1;
+            L.ctl->stall = -1;
         }
     }
     __syncthreads();
