@@ -130,6 +130,7 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
     const int ct = R->confine_type;
     const double cl = R->confine_length;
     res.overflow = 0;
+    const VoxGrid VG = vox_grid(R);
 
     for (int i = lane; i < BIG_HASH; i += WAVE) {
         S.hkey[i] = HKEY_EMPTY;
@@ -204,6 +205,7 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
         }
 
         // positions 2i (current) and 2i + 1 (trial), chunk-local
+        int ties = 0;
         if (act) {
             int32_t sp = 0;
 #pragma unroll
@@ -214,7 +216,7 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
                 for (int j = 0; j < 3; ++j) pt[j] = k == 0 ? o[j] : w[j];
                 int64_t ix[3];
                 double wt[3];
-                voxel_lower(R, pt, ix, wt);
+                voxel_lower(VG, pt, ix, wt, ties);
                 const int p = 2 * lane + k;
                 const int x = (int)ix[0], y = (int)ix[1], z = (int)ix[2];
                 S.prec[p] = make_int4(x | (wrap1(x, nx) << 16), y | (wrap1(y, ny) << 16),
@@ -223,6 +225,7 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
                 for (int j = 0; j < 3; ++j) S.pw[4 * p + j] = wt[j];
             }
         }
+        count_ties(R, ties);
         wave_sync();
 
         // hash the chunk's contributions; each slot collects the positions touching it
@@ -281,12 +284,12 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
                 }
             }
             int tcnt;
-            const int t = T + wave_excl_scan(first ? 1 : 0, tcnt);
+            const int t = T + wave_excl_count(first, tcnt);
             if (first) S.tlist[t] = h;
             T += tcnt;
             const bool fresh = first && S.hq[h] < 0;
             int ncnt;
-            const int q = U + wave_excl_scan(fresh ? 1 : 0, ncnt);
+            const int q = U + wave_excl_count(fresh, ncnt);
             if (fresh && q < S.ucap) {
                 S.hq[h] = q;
                 S.uvox[q] = S.hkey[h];
@@ -300,6 +303,25 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
             break;
         }
         wave_sync();
+        if (T > WAVE) {
+            // touched voxels longest fold first (fold_class): each round of the
+            // fold loop below then holds voxels of similar length
+            constexpr int J = BIG_E / WAVE;
+            int hh[J], cls[J], rank[J];
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+                const int t = lane + WAVE * j;
+                hh[j] = t < T ? S.tlist[t] : 0;
+                cls[j] = t < T ? fold_class(__popcll(S.hpos[hh[j]])) : 8;
+                rank[j] = 0;
+            }
+            wave_rank_by_class<J>(cls, rank);
+            wave_sync();
+#pragma unroll
+            for (int j = 0; j < J; ++j)
+                if (cls[j] < 8) S.tlist[rank[j]] = hh[j];
+            wave_sync();
+        }
 
         // fold: touched voxel t on lane t mod 64, its positions in increasing order
         for (int t0 = 0; t0 < T; t0 += WAVE) {

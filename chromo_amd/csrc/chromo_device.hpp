@@ -203,6 +203,40 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t m)
 }
 
 // Exclusive prefix sum over the wave; total receives the wave sum.
+// Exclusive prefix count of a predicate over the wavefront, and its total:
+// wave_excl_scan of 0 / 1 values from one ballot.  Whole wave.
+__device__ __forceinline__ int wave_excl_count(bool pred, int &total)
+{
+    const uint64_t m = __builtin_amdgcn_ballot_w64(pred);
+    total = __popcll(m);
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Fold-length classes, longest first: the positions a voxel's fold visits (1 or
+// 2 -> 7, ..., 15 or more -> 0).
+__device__ __forceinline__ int fold_class(int visits) { return visits >= 15 ? 0 : (16 - visits) >> 1; }
+
+// Ranks of the lane's J items (item j of lane l is item l + 64 j) in the order
+// (class, item) for classes 0..7; items of class 8 or more get no rank.  Items
+// dealt out to lanes in that order give each round of a per-voxel fold loop
+// voxels of similar length: the loop runs as long as its longest voxel.
+// Deterministic.  Whole wave.
+template <int J>
+__device__ __forceinline__ void wave_rank_by_class(const int *cls, int *rank)
+{
+    int off = 0;
+    for (int c = 0; c < 8; ++c) {
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const uint64_t m = __builtin_amdgcn_ballot_w64(cls[j] == c);
+            if (cls[j] == c)
+                rank[j] = off + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            off += __popcll(m);
+        }
+    }
+}
+
 __device__ __forceinline__ int wave_excl_scan(int x, int &total)
 {
     // inclusive scan within rows of 16 (lanes shifted in from outside the row read 0)
@@ -953,46 +987,102 @@ __device__ __forceinline__ double field_dE_total(const chromo_replica *R, double
 // Field geometry (chromo/fields.pyx:1434-1673)
 // ---------------------------------------------------------------------------
 
-// Lower voxel indices and lower-corner weights of one position.
-__device__ __forceinline__ void voxel_lower(const chromo_replica *R, const double *pos, int64_t ix[3],
-                                            double w[3])
+// The grid fields voxel_lower reads, taken out of the replica record once per
+// evaluation: the record lives in LDS, and a field read there after the
+// evaluation's own LDS stores would be read again at every use.
+struct VoxGrid {
+    double width[3], half_width[3], half_step[3], dxyz[3];
+    int64_t n[3];
+    int fast;
+    double half_points, sub_width[3];
+    int64_t n_sub[3];
+};
+
+__device__ __forceinline__ VoxGrid vox_grid(const chromo_replica *R)
 {
-    const int64_t nn[3] = {R->nx, R->ny, R->nz};
-    if (R->fast_field) {
+    VoxGrid G;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        G.width[j] = R->width[j];
+        G.half_width[j] = R->half_width[j];
+        G.half_step[j] = R->half_step[j];
+        G.dxyz[j] = R->dxyz[j];
+        G.sub_width[j] = R->sub_width[j];
+        G.n_sub[j] = R->n_sub[j];
+    }
+    G.n[0] = R->nx;
+    G.n[1] = R->ny;
+    G.n[2] = R->nz;
+    G.fast = R->fast_field;
+    G.half_points = R->half_points;
+    return G;
+}
+
+// Lower voxel indices and lower-corner weights of one position.
+//
+// `ties` gains one per coordinate within TIE_VOXEL_EPS voxel of a voxel plane,
+// where an ulp of difference in the position (device sin / cos / acos / sqrt
+// against glibc's) could change the floor.  The caller adds the count to
+// CHROMO_CNT_TIE_VOXEL; keeping the atomic out of here lets the three axes (and
+// a caller's two positions) run as independent chains.
+__device__ __forceinline__ void voxel_lower(const VoxGrid &G, const double *pos, int64_t ix[3], double w[3],
+                                            int &ties)
+{
+    if (G.fast) {
         // the position's sub-bin floor((x + W/2) / sbw) mod (n_points n) and that
         // sub-bin's lower voxel and weight, from init_fast_field's expressions
         // (fields.pyx:614-671, 1255-1300)
-        const double half = R->half_points;
+#pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const double sw = R->sub_width[j];
-            const double qs = (pos[j] + R->half_width[j]) / sw;
+            const double sw = G.sub_width[j];
+            const double qs = (pos[j] + G.half_width[j]) / sw;
             const double sb = isfinite(qs) ? floor(qs) : 0.0;
-            if (fabs(qs - rint(qs)) < TIE_VOXEL_EPS)
-                atomicAdd((unsigned long long *)&R->counters[CHROMO_CNT_TIE_VOXEL], 1ull);
-            int64_t i = (fabs(sb) < 0x1p62) ? (int64_t)sb % R->n_sub[j] : 0;
-            if (i < 0) i += R->n_sub[j];
-            const double u = (((double)i - half) * sw) / R->dxyz[j];
+            ties += fabs(qs - rint(qs)) < TIE_VOXEL_EPS;
+            int64_t i = (fabs(sb) < 0x1p62) ? (int64_t)sb % G.n_sub[j] : 0;
+            if (i < 0) i += G.n_sub[j];
+            const double u = (((double)i - G.half_points) * sw) / G.dxyz[j];
             const double ind = floor(u);
-            ix[j] = (ind == -1.0) ? nn[j] - 1 : (int64_t)ind;
+            ix[j] = (ind == -1.0) ? G.n[j] - 1 : (int64_t)ind;
             w[j] = 1 - (u - ind);
         }
         return;
     }
+#pragma unroll
     for (int j = 0; j < 3; ++j) {
-        double x = pymod(pos[j] + R->half_width[j], R->width[j]) - R->half_step[j];
-        double q = x / R->dxyz[j];
+        double x = pymod(pos[j] + G.half_width[j], G.width[j]) - G.half_step[j];
+        double q = x / G.dxyz[j];
         // finite coordinates give q in [-0.5, n - 0.5); a non-finite one is pinned to
         // voxel 0 so that no index can leave the grid
-        if (!(q >= -1.0 && q < (double)nn[j])) q = 0.0;
-        int64_t ind = (int64_t)floor(q);
-        ix[j] = (ind == -1) ? nn[j] - 1 : ind;
-        w[j] = 1 - (q - (double)ind);
-        // within 1e-11 voxel of a voxel plane, an ulp of difference in the position
-        // (device sin / cos / acos / sqrt against glibc's) could change the floor:
-        // counted, one atomic per (rare) event
-        if (fabs(q - rint(q)) < TIE_VOXEL_EPS)
-            atomicAdd((unsigned long long *)&R->counters[CHROMO_CNT_TIE_VOXEL], 1ull);
+        if (!(q >= -1.0 && q < (double)G.n[j])) q = 0.0;
+        const double fq = floor(q);
+        ix[j] = (fq == -1.0) ? G.n[j] - 1 : (int64_t)(int32_t)fq;
+        w[j] = 1 - (q - fq);
+        ties += fabs(q - rint(q)) < TIE_VOXEL_EPS;
     }
+}
+
+__device__ __forceinline__ void voxel_lower(const chromo_replica *R, const double *pos, int64_t ix[3],
+                                            double w[3], int &ties)
+{
+    voxel_lower(vox_grid(R), pos, ix, w, ties);
+}
+
+// voxel_lower for one lane on its own: the tie count goes straight to the counter.
+__device__ __forceinline__ void voxel_lower_1(const chromo_replica *R, const double *pos, int64_t ix[3],
+                                              double w[3])
+{
+    int ties = 0;
+    voxel_lower(R, pos, ix, w, ties);
+    if (ties) atomicAdd((unsigned long long *)&R->counters[CHROMO_CNT_TIE_VOXEL], (unsigned long long)ties);
+}
+
+// Adds the lanes' tie counts to CHROMO_CNT_TIE_VOXEL with one atomic.  Whole wave.
+__device__ __forceinline__ void count_ties(const chromo_replica *R, int ties)
+{
+    if (!wave_any(ties != 0)) return;
+    int tot;
+    wave_excl_scan(ties, tot);
+    if (lane_id() == 0) atomicAdd((unsigned long long *)&R->counters[CHROMO_CNT_TIE_VOXEL], (unsigned long long)tot);
 }
 
 // Voxel super-index of corner l (x fastest) with the wrap of

@@ -281,7 +281,7 @@ __device__ __forceinline__ void field_tape_body(const chromo_replica *R, const i
             const double *pos = (k == 0 || state_change) ? o : w;
             int64_t ix[3];
             double wt[3];
-            voxel_lower(R, pos, ix, wt);
+            voxel_lower_1(R, pos, ix, wt);
             const int p = (int)(2 * i + k);
             for (int l = 0; l < 8; ++l) {
                 const int64_t cv = corner_voxel(R, ix, l);

@@ -348,7 +348,7 @@ __device__ __forceinline__ void move_body(const chromo_replica *R, double u, Pro
                 for (int j = 0; j < 3; ++j) pos[j] = (k == 0 || state_change) ? o[j] : w[j];
                 int64_t ix[3];
                 double wt[3];
-                voxel_lower(R, pos, ix, wt);
+                voxel_lower_1(R, pos, ix, wt);
                 const int p = (int)(2 * i + k);
                 const double vol_bin = R->vol_bin;
                 // binder states of this configuration, one byte per binder

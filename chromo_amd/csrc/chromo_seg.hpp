@@ -157,6 +157,58 @@ __device__ __forceinline__ void fold_c0(double c0, double pre, int32_t sq, doubl
     }
 }
 
+// One entry of a voxel's fold list: the contribution's pre * c0 (pre = +-1, so
+// exactly fold_c0's t0, and c * st exactly its pre * (c0 * st)) and the binder
+// states of its position.
+struct alignas(16) FoldEntry {
+    double c;
+    int32_t st, pad;
+};
+
+template <int NB>
+__device__ __forceinline__ void fold_entry(const FoldEntry &f, double *acc, bool thr)
+{
+    if (!thr || fabs(f.c) > 1E-18) acc[0] += f.c;
+#pragma unroll
+    for (int m = 1; m <= NB; ++m) {
+        const double t = f.c * (double)((f.st >> (8 * (m - 1))) & 0xff);
+        if (!thr || fabs(t) > 1E-18) acc[m] += t;
+    }
+}
+
+// Contribution e of position p goes to entry off(q) + (positions of q's mask
+// below p) of its voxel q's fold list (hmin maps its hash slot to q, ucrd.y's
+// upper half holds off(q)).  The arrays are distinct LDS regions: with no
+// aliasing to fear, the rounds' loads need not wait for earlier rounds' stores.
+template <int MW, int ROUNDS>
+__device__ __forceinline__ void seg_build_lists(int E, int rounds, const int32_t *__restrict__ eslot,
+                                                const int32_t *__restrict__ hmin, const uint64_t *__restrict__ umask,
+                                                const int2 *__restrict__ ucrd, const double *__restrict__ c0,
+                                                const int4 *__restrict__ prec, FoldEntry *__restrict__ list)
+{
+    const int lane = lane_id();
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        if (r >= rounds) break;
+        const int e = lane + WAVE * r;
+        if (e < E) {
+            const int p = e >> 3;
+            const int q = hmin[eslot[e]];
+            int k = 0;
+#pragma unroll
+            for (int w = 0; w < MW; ++w) {
+                const uint64_t mk = umask[q * MW + w];
+                k += (w < (p >> 6)) ? __popcll(mk) : (w == (p >> 6) ? __popcll(mk & ((1ull << (p & 63)) - 1)) : 0);
+            }
+            FoldEntry f;
+            f.c = ((p & 1) ? 1. : -1.) * c0[e];
+            f.st = prec[p].w;
+            f.pad = 0;
+            list[(ucrd[q].y >> 16) + k] = f;
+        }
+    }
+}
+
 // Trial coordinates of bead (o) under proposal P, lane-local (move_funcs.pyx:139-154).
 __device__ __forceinline__ void seg_transform(const Proposal &P, const double o[9], double w[9])
 {
@@ -404,6 +456,8 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
         const double *avol = R->access_vol;
         const bool thr = R->fast_field == 0;
         // positions 2i (current) and 2i + 1 (trial; the current one for a state change)
+        int ties = 0;
+        const VoxGrid VG = vox_grid(R);
         if (act) {
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
@@ -411,7 +465,7 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
                 for (int j = 0; j < 3; ++j) pt[j] = (k == 0 || state_change) ? o[j] : w[j];
                 int64_t ix[3];
                 double wt[3];
-                voxel_lower(R, pt, ix, wt);
+                voxel_lower(VG, pt, ix, wt, ties);
                 const int p = 2 * lane + k;
                 const int x = (int)ix[0], y = (int)ix[1], z = (int)ix[2];
                 int32_t sp = 0;
@@ -424,6 +478,7 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
                 for (int j = 0; j < 3; ++j) S.pw[4 * p + j] = wt[j];
             }
         }
+        count_ties(R, ties);
         wave_sync();
 
         // 1. unique voxels: hash the contributions in order, keep each voxel's first
@@ -465,7 +520,7 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
             const int h = e < E ? S.eslot[e] : 0;
             const bool first = e < E && S.hmin[h] == e;
             int cnt;
-            const int q = total + wave_excl_scan(first ? 1 : 0, cnt);
+            const int q = total + wave_excl_count(first, cnt);
             if (first && q < SEG_UMAX) {
                 const int4 pr = S.prec[e >> 3];
                 const int l = e & 7;
@@ -482,13 +537,21 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
         }
         U = total;
         wave_sync();
+        // Unless a grid axis has one voxel (a position's corners can then share a
+        // voxel), each voxel's contributions are laid out below as one list, in
+        // fold order, in the position-mask area of the hash; its hmin slots map
+        // hash slot -> voxel meanwhile, and both are emptied after the fold.
+        const bool degen = nx == 1 || ny == 1 || nz == 1;
+        const bool listed = !degen && U <= SEG_UMAX;
         // release the hash (every key is some voxel's first occurrence)
         for (int q = lane; q < U && q < SEG_UMAX; q += WAVE) {
             const int h = S.uaux[q];
             S.hkey[h] = HKEY_EMPTY;
-            S.hmin[h] = HMIN_EMPTY;
+            if (!listed) {
+                S.hmin[h] = HMIN_EMPTY;
 #pragma unroll
-            for (int w = 0; w < MW; ++w) S.hpos[h * MW + w] = 0;
+                for (int w = 0; w < MW; ++w) S.hpos[h * MW + w] = 0;
+            }
         }
         if (U > SEG_UMAX) {
             // too many voxels for the registers: the global-arena path evaluates it
@@ -505,9 +568,45 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
             return;
         }
         wave_sync();
+        if (U > WAVE) {
+            // deal the voxels out longest fold first (fold_class), so that each
+            // round of the fold loop below holds voxels of similar length
+            int vv[SEG_VB], hh[SEG_VB], cls[SEG_VB], rank[SEG_VB];
+            int2 cc[SEG_VB];
+            uint64_t mm[SEG_VB][MW];
+#pragma unroll
+            for (int j = 0; j < SEG_VB; ++j) {
+                const int q = lane + WAVE * j;
+                const bool own = q < U;
+                vv[j] = own ? S.uvox[q] : 0;
+                hh[j] = own ? S.uaux[q] : 0;
+                cc[j] = own ? S.ucrd[q] : make_int2(0, 0);
+                int visits = 0;
+#pragma unroll
+                for (int w = 0; w < MW; ++w) {
+                    mm[j][w] = own ? S.umask[q * MW + w] : 0;
+                    visits += __popcll(mm[j][w]);
+                }
+                cls[j] = own ? fold_class(visits) : 8;
+                rank[j] = 0;
+            }
+            wave_rank_by_class<SEG_VB>(cls, rank);
+            wave_sync();
+#pragma unroll
+            for (int j = 0; j < SEG_VB; ++j) {
+                if (cls[j] < 8) {
+                    const int r = rank[j];
+                    S.uvox[r] = vv[j];
+                    S.uaux[r] = hh[j];
+                    S.ucrd[r] = cc[j];
+#pragma unroll
+                    for (int w = 0; w < MW; ++w) S.umask[r * MW + w] = mm[j][w];
+                }
+            }
+            wave_sync();
+        }
 
         // 2. fold per voxel, voxel lane + 64 j on this lane
-        const bool degen = nx == 1 || ny == 1 || nz == 1;
         const GLOBAL_AS double *rho_g = gmem(R->density);
         const int npos = (int)(2 * n);
         double S2[NB], SX[NB * NB], NS = 0;
@@ -515,76 +614,150 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
         for (int j = 0; j < NB; ++j) S2[j] = 0;
 #pragma unroll
         for (int j = 0; j < NB * NB; ++j) SX[j] = 0;
+        FoldEntry *list = (FoldEntry *)S.hpos;
+        if (listed) {
+            // list offsets: voxels in (j, lane) order, each as long as its position mask;
+            // the offset rides in the upper half of ucrd.y (z < 2^15)
+            int base = 0;
 #pragma unroll
-        for (int j = 0; j < SEG_VB; ++j) {
+            for (int j = 0; j < SEG_VB; ++j) {
+                if (WAVE * j >= U) break;
+                const int q = lane + WAVE * j;
+                const bool own = q < U;
+                int visits = 0;
 #pragma unroll
-            for (int m = 0; m < Bp1; ++m) K.nr[j][m] = 0;
-            if (WAVE * j >= U) continue;
-            const int q = lane + WAVE * j;
-            const bool own = q < U;
-            double rho[Bp1], acc[Bp1];
-#pragma unroll
-            for (int m = 0; m < Bp1; ++m) {
-                rho[m] = 0;
-                acc[m] = 0;
-            }
-            int vx = -1, vy = -1, vz = -1;
-            uint64_t hits[MW];
-#pragma unroll
-            for (int w = 0; w < MW; ++w) hits[w] = 0;
-            int vown = 0;
-            if (own) {
-                const int v = S.uvox[q];
-                vown = v;
-#pragma unroll
-                for (int m = 0; m < Bp1; ++m) rho[m] = rho_g[(int64_t)v * Bp1 + m];
-                const int2 c = S.ucrd[q];
-                vx = c.x & 0xffff;
-                vy = c.x >> 16;
-                vz = c.y;
-#pragma unroll
-                for (int w = 0; w < MW; ++w) hits[w] = S.umask[q * MW + w];
-            }
-            // each lane visits the positions that touch its voxel, in order
-            for (;;) {
-                bool more = false;
-#pragma unroll
-                for (int w = 0; w < MW; ++w) more |= hits[w] != 0;
-                if (!wave_any(more)) break;
-                if (!more) continue;
-                int p = 0;
-#pragma unroll
-                for (int w = MW - 1; w >= 0; --w)
-                    if (hits[w] != 0) p = 64 * w + __builtin_ctzll(hits[w]);
-#pragma unroll
-                for (int w = 0; w < MW; ++w)
-                    if (w == (p >> 6)) hits[w] &= hits[w] - 1;
-                const int4 pr = S.prec[p];
-                const double pre = (p & 1) ? 1. : -1.;
-                const int x0 = pr.x & 0xffff, x1 = pr.x >> 16, y0 = pr.y & 0xffff, y1 = pr.y >> 16,
-                          z0 = pr.z & 0xffff, z1 = pr.z >> 16;
-                if (!degen) {
-                    const bool hx = vx == x0 || vx == x1, hy = vy == y0 || vy == y1, hz = vz == z0 || vz == z1;
-                    if (hx && hy && hz) {
-                        const int l = (vx == x1 ? 1 : 0) | (vy == y1 ? 2 : 0) | (vz == z1 ? 4 : 0);
-                        fold_c0<NB>(S.c0[p * 8 + l], pre, pr.w, acc, thr);
-                    }
-                } else {
-                    int dx = vx - x0, dy = vy - y0, dz = vz - z0;
-                    if (dx < 0) dx += nx;
-                    if (dy < 0) dy += ny;
-                    if (dz < 0) dz += nz;
-                    if (own)
-                        for (int l = 0; l < 8; ++l)
-                            if (axis_hit(l & 1, dx, nx) && axis_hit((l >> 1) & 1, dy, ny) &&
-                                axis_hit((l >> 2) & 1, dz, nz))
-                                fold_c0<NB>(S.c0[p * 8 + l], pre, pr.w, acc, thr);
+                for (int w = 0; w < MW; ++w) visits += own ? __popcll(S.umask[q * MW + w]) : 0;
+                int tot;
+                const int off = base + wave_excl_scan(visits, tot);
+                base += tot;
+                if (own) {
+                    S.hmin[S.uaux[q]] = q;
+                    S.ucrd[q].y |= off << 16;
                 }
             }
-            if (own) {
-                voxel_dE_terms<NB>(R, vown, rho, acc, S2, SX, NS);
+            wave_sync();
+            seg_build_lists<MW, C::ROUNDS>(E, rounds, S.eslot, S.hmin, S.umask, S.ucrd, S.c0, S.prec, list);
+            wave_sync();
+        }
+        if (listed) {
 #pragma unroll
-                for (int m = 0; m < Bp1; ++m) K.nr[j][m] = rho[m] + acc[m];
+            for (int j = 0; j < SEG_VB; ++j) {
+#pragma unroll
+                for (int m = 0; m < Bp1; ++m) K.nr[j][m] = 0;
+                if (WAVE * j >= U) continue;
+                const int q = lane + WAVE * j;
+                const bool own = q < U;
+                double rho[Bp1], acc[Bp1];
+#pragma unroll
+                for (int m = 0; m < Bp1; ++m) {
+                    rho[m] = 0;
+                    acc[m] = 0;
+                }
+                int vown = 0, cnt = 0, off = 0;
+                if (own) {
+                    vown = S.uvox[q];
+#pragma unroll
+                    for (int m = 0; m < Bp1; ++m) rho[m] = rho_g[(int64_t)vown * Bp1 + m];
+#pragma unroll
+                    for (int w = 0; w < MW; ++w) cnt += __popcll(S.umask[q * MW + w]);
+                    off = S.ucrd[q].y >> 16;
+                }
+                // the voxel's contributions in the reference's order, two entries
+                // requested at a time
+                for (int k = 0; wave_any(k < cnt); k += 2) {
+                    FoldEntry f0 = {0., 0, 0}, f1 = {0., 0, 0};
+                    if (k < cnt) f0 = list[off + k];
+                    if (k + 1 < cnt) f1 = list[off + k + 1];
+                    if (k < cnt) fold_entry<NB>(f0, acc, thr);
+                    if (k + 1 < cnt) fold_entry<NB>(f1, acc, thr);
+                }
+                if (own) {
+                    voxel_dE_terms<NB>(R, vown, rho, acc, S2, SX, NS);
+#pragma unroll
+                    for (int m = 0; m < Bp1; ++m) K.nr[j][m] = rho[m] + acc[m];
+                }
+            }
+            // empty the hash again: the hmin map, the used slots' masks and the lists
+            for (int q = lane; q < U; q += WAVE) {
+                const int h = S.uaux[q];
+                S.hmin[h] = HMIN_EMPTY;
+#pragma unroll
+                for (int w = 0; w < MW; ++w) S.hpos[h * MW + w] = 0;
+            }
+            for (int i = lane; i < E; i += WAVE) list[i] = FoldEntry{0., 0, 0};
+            wave_sync();
+        } else {
+#pragma unroll
+            for (int j = 0; j < SEG_VB; ++j) {
+#pragma unroll
+                for (int m = 0; m < Bp1; ++m) K.nr[j][m] = 0;
+                if (WAVE * j >= U) continue;
+                const int q = lane + WAVE * j;
+                const bool own = q < U;
+                double rho[Bp1], acc[Bp1];
+#pragma unroll
+                for (int m = 0; m < Bp1; ++m) {
+                    rho[m] = 0;
+                    acc[m] = 0;
+                }
+                int vx = -1, vy = -1, vz = -1;
+                uint64_t hits[MW];
+#pragma unroll
+                for (int w = 0; w < MW; ++w) hits[w] = 0;
+                int vown = 0;
+                if (own) {
+                    const int v = S.uvox[q];
+                    vown = v;
+#pragma unroll
+                    for (int m = 0; m < Bp1; ++m) rho[m] = rho_g[(int64_t)v * Bp1 + m];
+                    const int2 c = S.ucrd[q];
+                    vx = c.x & 0xffff;
+                    vy = c.x >> 16;
+                    vz = c.y;
+#pragma unroll
+                    for (int w = 0; w < MW; ++w) hits[w] = S.umask[q * MW + w];
+                }
+                // each lane visits the positions that touch its voxel, in order
+                for (;;) {
+                    bool more = false;
+#pragma unroll
+                    for (int w = 0; w < MW; ++w) more |= hits[w] != 0;
+                    if (!wave_any(more)) break;
+                    if (!more) continue;
+                    int p = 0;
+#pragma unroll
+                    for (int w = MW - 1; w >= 0; --w)
+                        if (hits[w] != 0) p = 64 * w + __builtin_ctzll(hits[w]);
+#pragma unroll
+                    for (int w = 0; w < MW; ++w)
+                        if (w == (p >> 6)) hits[w] &= hits[w] - 1;
+                    const int4 pr = S.prec[p];
+                    const double pre = (p & 1) ? 1. : -1.;
+                    const int x0 = pr.x & 0xffff, x1 = pr.x >> 16, y0 = pr.y & 0xffff, y1 = pr.y >> 16,
+                              z0 = pr.z & 0xffff, z1 = pr.z >> 16;
+                    if (!degen) {
+                        const bool hx = vx == x0 || vx == x1, hy = vy == y0 || vy == y1, hz = vz == z0 || vz == z1;
+                        if (hx && hy && hz) {
+                            const int l = (vx == x1 ? 1 : 0) | (vy == y1 ? 2 : 0) | (vz == z1 ? 4 : 0);
+                            fold_c0<NB>(S.c0[p * 8 + l], pre, pr.w, acc, thr);
+                        }
+                    } else {
+                        int dx = vx - x0, dy = vy - y0, dz = vz - z0;
+                        if (dx < 0) dx += nx;
+                        if (dy < 0) dy += ny;
+                        if (dz < 0) dz += nz;
+                        if (own)
+                            for (int l = 0; l < 8; ++l)
+                                if (axis_hit(l & 1, dx, nx) && axis_hit((l >> 1) & 1, dy, ny) &&
+                                    axis_hit((l >> 2) & 1, dz, nz))
+                                    fold_c0<NB>(S.c0[p * 8 + l], pre, pr.w, acc, thr);
+                    }
+                }
+                if (own) {
+                    voxel_dE_terms<NB>(R, vown, rho, acc, S2, SX, NS);
+#pragma unroll
+                    for (int m = 0; m < Bp1; ++m) K.nr[j][m] = rho[m] + acc[m];
+                }
             }
         }
         const double share = wave_sum(lane_field_share<NB>(R, S2, SX, NS));
