@@ -344,3 +344,20 @@ def test_twist_absolute_energy(oracle):
     m = EnergyModel(c, s.field, s.binders)
     with pytest.raises(L.ChromoError, match="twist"):
         energy(m, torch.as_tensor(c.r), torch.as_tensor(m.theta0))
+
+
+# ------------------------------------------------ long windows, two wavefronts ---
+
+@pytest.mark.parametrize("rng_mode", [L.RNG_PHILOX, L.RNG_GLIBC], ids=["philox", "glibc"])
+def test_long_window_blocks_two_wavefronts(oracle, rng_mode):
+    """Steady-state amplitudes (crank-shaft, slide and end-pivot windows of 100-150
+    beads) with two wavefronts: about a third of those proposals exceed the
+    speculative evaluator and run as sequential slots (the 64-bead and chunked
+    evaluators) between speculative rounds; the chain is the oracle's."""
+    say("long windows, W=2", rng_mode)
+    s = W.make_system(6000, 2, 24, seed=97, eps=[-4, -2], mu=[-0.4, 0.3])
+    for c in s.ctrls:
+        if c.move in (L.CRANK_SHAFT, L.SLIDE, L.END_PIVOT):
+            c.amp_bead, c.bead_lo, c.bead_hi = 130, 100.0, 150.0
+    lo, _ = _run_pair(oracle, s, 6, rng_mode=rng_mode, waves=2, rel=1e-9)
+    assert np.any(lo["n"] > 64)
