@@ -162,12 +162,6 @@ __global__ void k_dens_sort(int64_t nbins, const int32_t *off, const int32_t *cn
     }
 }
 
-__global__ void k_fill_i32(int32_t *p, int64_t n)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-
 // The buckets that reach voxel v: cell[l] holds the beads of which v is corner l;
 // a bucket reached by several corners (grids one or two voxels wide) is kept once,
 // in its first slot, whose mask collects every such corner.
@@ -312,11 +306,74 @@ k_dens_gather(GridGeom G, int Bp1, const int32_t *off, const int32_t *cnt, const
     }
 }
 
+// The bead records in bucket order (brec_sorted[i] = brec[list[i]]): a bucket's
+// beads are then one contiguous run.
+__global__ void k_dens_permute(int64_t N, const int32_t *list, const BeadRec *brec, BeadRec *brec_sorted)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) brec_sorted[i] = brec[list[i]];
+}
+
+// The density of the energy adjoint, which needs a deterministic sum but not the
+// reference's bead-order fold: voxel v sums each of its (<= 8) sorted buckets on
+// its own, the buckets advancing together one bead per step (independent chains,
+// so their loads overlap; each a contiguous run of brec_sorted), then adds the
+// bucket sums in corner order.  A step per bead of the longest bucket, where the
+// bead-order merge takes one per bead of all eight.
+template <int NB>
+__global__ void __launch_bounds__(256)
+k_dens_gather_fixed(GridGeom G, const int32_t *off, const int32_t *cnt, const BeadRec *brec_sorted,
+                    double *density)
+{
+    constexpr int Bp1 = NB + 1;
+    const int64_t nbv = G.nx * G.ny * G.nz;
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= nbv) return;
+    int32_t cell[8], pos[8], n[8];
+    int msk[8];
+    voxel_cells(G, v, cell, msk);
+    const double vvol = G.access_vol ? G.access_vol[v] : G.vol_bin;
+    int kmax = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        pos[k] = msk[k] ? off[cell[k]] : 0;
+        n[k] = msk[k] ? cnt[cell[k]] : 0;
+        kmax = n[k] > kmax ? n[k] : kmax;
+    }
+    double acc[8][Bp1];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int m = 0; m < Bp1; ++m) acc[k][m] = 0;
+    for (int i = 0; i < kmax; ++i) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (i >= n[k]) continue;
+            const BeadRec q = brec_sorted[pos[k] + i];
+#pragma unroll
+            for (int l = 0; l < 8; ++l) {
+                if (!(msk[k] >> l & 1)) continue;
+                const double d = corner_weight(l, q.w[0], q.w[1], q.w[2]) / vvol;
+                acc[k][0] += d;
+#pragma unroll
+                for (int m = 1; m < Bp1; ++m) acc[k][m] += d * (double)q.st[m - 1];
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < Bp1; ++m) {
+        double a = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a += acc[k][m];
+        density[v * Bp1 + m] = a;
+    }
+}
+
 // Scratch for one scatter: sized by the largest chain and grid.
 struct DensityScratch {
     int32_t *cell = nullptr, *list = nullptr, *cnt = nullptr, *off = nullptr, *fill = nullptr;
     int32_t *bsum = nullptr, *boff = nullptr, *bsum2 = nullptr;
-    BeadRec *brec = nullptr;
+    BeadRec *brec = nullptr, *brec_sorted = nullptr;
     int64_t maxN = 0, maxBins = 0;
 };
 
@@ -325,8 +382,10 @@ constexpr size_t SCRATCH_ALIGN = 256;
 inline size_t align_scratch(size_t o) { return (o + SCRATCH_ALIGN - 1) / SCRATCH_ALIGN * SCRATCH_ALIGN; }
 
 // The scatter's scratch carved from one buffer at base (nullptr: sizes only);
-// returns the bytes used, a multiple of SCRATCH_ALIGN.
-inline size_t density_scratch_carve(DensityScratch &S, char *base, int64_t maxN, int64_t maxBins)
+// returns the bytes used, a multiple of SCRATCH_ALIGN.  sorted_records: room for
+// the bucket-ordered bead records of the fixed-order gather.
+inline size_t density_scratch_carve(DensityScratch &S, char *base, int64_t maxN, int64_t maxBins,
+                                    bool sorted_records)
 {
     size_t o = 0;
     auto take = [&](size_t bytes) {
@@ -339,6 +398,7 @@ inline size_t density_scratch_carve(DensityScratch &S, char *base, int64_t maxN,
     S.cell = (int32_t *)take(sizeof(int32_t) * maxN);
     S.list = (int32_t *)take(sizeof(int32_t) * maxN);
     S.brec = (BeadRec *)take(sizeof(BeadRec) * maxN);
+    S.brec_sorted = sorted_records ? (BeadRec *)take(sizeof(BeadRec) * maxN) : nullptr;
     S.cnt = (int32_t *)take(sizeof(int32_t) * maxBins);
     S.off = (int32_t *)take(sizeof(int32_t) * maxBins);
     S.fill = (int32_t *)take(sizeof(int32_t) * maxBins);
@@ -375,14 +435,28 @@ inline void density_scratch_free(DensityScratch &S, hipStream_t st)
     for (auto p : ip)
         if (p) (void)hipFreeAsync(p, st);
     if (S.brec) (void)hipFreeAsync(S.brec, st);
+    if (S.brec_sorted) (void)hipFreeAsync(S.brec_sorted, st);
     S = DensityScratch();
 }
 
+// Clears two int32 arrays of n elements.
+__global__ void k_fill2_i32(int32_t *p, int32_t *q, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        p[i] = 0;
+        q[i] = 0;
+    }
+}
+
 // Scatter N beads into density (n_bins x Bp1).  Requires N <= S.maxN,
-// n_bins <= S.maxBins <= 2048 * 2048.
+// n_bins <= S.maxBins <= 2048 * 2048.  fixed_order: the adjoint's deterministic
+// corner-by-corner sum (k_dens_gather_fixed; S carved with sorted_records) instead
+// of the reference's bead-order fold; cleanup (the 1e-18 rule of the initial
+// densities) applies to the fold only.
 inline hipError_t density_scatter(const GridGeom &G, int64_t N, int Bp1, PosView P,
                                   const int8_t *states, double *density, int cleanup,
-                                  DensityScratch &S, hipStream_t st)
+                                  DensityScratch &S, hipStream_t st, bool fixed_order = false)
 {
     const int64_t nb = G.nx * G.ny * G.nz;
     hipError_t e;
@@ -390,8 +464,7 @@ inline hipError_t density_scatter(const GridGeom &G, int64_t N, int Bp1, PosView
     unsigned gv = (unsigned)((nb + 255) / 256);
     // cleared by a kernel rather than hipMemsetAsync: a caller may capture this
     // sequence in a HIP graph
-    hipLaunchKernelGGL(k_fill_i32, dim3(gv), dim3(256), 0, st, S.cnt, nb);
-    hipLaunchKernelGGL(k_fill_i32, dim3(gv), dim3(256), 0, st, S.fill, nb);
+    hipLaunchKernelGGL(k_fill2_i32, dim3(gv), dim3(256), 0, st, S.cnt, S.fill, nb);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_dens_cells, dim3(gb), dim3(256), 0, st, G, N, P, Bp1 - 1, states, S.cell, S.brec,
                        S.cnt);
@@ -401,7 +474,15 @@ inline hipError_t density_scatter(const GridGeom &G, int64_t N, int Bp1, PosView
     hipLaunchKernelGGL(k_scan_add, dim3(gv), dim3(256), 0, st, S.off, S.boff, nb);
     hipLaunchKernelGGL(k_dens_place, dim3(gb), dim3(256), 0, st, N, S.cell, S.off, S.fill, S.list);
     hipLaunchKernelGGL(k_dens_sort, dim3(gv), dim3(256), 0, st, nb, S.off, S.cnt, S.list);
-    if (nb >= (int64_t)1024 * GATHER_BIG_BLOCKS)
+    if (fixed_order) {
+        hipLaunchKernelGGL(k_dens_permute, dim3(gb), dim3(256), 0, st, N, S.list, S.brec, S.brec_sorted);
+        switch (Bp1 - 1) {
+        case 1: hipLaunchKernelGGL(k_dens_gather_fixed<1>, dim3(gv), dim3(256), 0, st, G, S.off, S.cnt, S.brec_sorted, density); break;
+        case 2: hipLaunchKernelGGL(k_dens_gather_fixed<2>, dim3(gv), dim3(256), 0, st, G, S.off, S.cnt, S.brec_sorted, density); break;
+        case 3: hipLaunchKernelGGL(k_dens_gather_fixed<3>, dim3(gv), dim3(256), 0, st, G, S.off, S.cnt, S.brec_sorted, density); break;
+        default: hipLaunchKernelGGL(k_dens_gather_fixed<4>, dim3(gv), dim3(256), 0, st, G, S.off, S.cnt, S.brec_sorted, density); break;
+        }
+    } else if (nb >= (int64_t)1024 * GATHER_BIG_BLOCKS)
         hipLaunchKernelGGL((k_dens_gather<1024, true>), dim3((unsigned)((nb + 1023) / 1024)), dim3(1024), 0, st, G, Bp1,
                            S.off, S.cnt, S.list, S.brec, density, cleanup);
     else

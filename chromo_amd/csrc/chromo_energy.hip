@@ -579,11 +579,13 @@ struct EnergyBufs {
 
 // Carves the buffers of an evaluation of M from base (nullptr: sizes only) and
 // returns the bytes used.  rho_given: the density goes to the caller's array.
-size_t energy_carve(const EModel &M, bool grad, bool rho_given, char *base, EnergyBufs &Bf)
+// fixed_order: the adjoint's density (k_dens_gather_fixed); otherwise the
+// reference's bead-order fold.
+size_t energy_carve(const EModel &M, bool grad, bool rho_given, bool fixed_order, char *base, EnergyBufs &Bf)
 {
     const int64_t nb = M.G.nx * M.G.ny * M.G.nz;
     const int Bp1 = M.B + 1;
-    size_t o = density_scratch_carve(Bf.S, base, M.N, nb);
+    size_t o = density_scratch_carve(Bf.S, base, M.N, nb, fixed_order);
     auto take = [&](size_t bytes) {
         double *p = base ? (double *)(base + o) : nullptr;
         o = align_scratch(o + bytes);
@@ -614,7 +616,9 @@ int energy_adjoint_run(const EModel &M, const double *d_r, const double *d_theta
 {
     const int B = M.B, Bp1 = B + 1;
     PosView P{d_r, 1, 3};
-    hipError_t e = density_scatter(M.G, M.N, Bp1, P, M.states, Bf.rho, 0, Bf.S, st);
+    // a deterministic density, not the reference's bead-order fold (nothing compares
+    // the adjoint's densities with the reference's bit for bit)
+    hipError_t e = density_scatter(M.G, M.N, Bp1, P, M.states, Bf.rho, 0, Bf.S, st, true);
     if (e != hipSuccess) return eset(-2, std::string("density scatter: ") + hipGetErrorString(e));
     const int ntv = B + B * (B - 1) / 2 + 2, ntb = 2 + 2 * B;
     const int gv = Bf.gv, gb = Bf.gb;
@@ -678,11 +682,11 @@ int chromo_energy_adjoint(const chromo_energy_model *model, const double *d_r, c
     if (!d_r || !d_theta || !d_E) return eset(-1, "NULL argument");
     hipStream_t st = (hipStream_t)stream;
     EnergyBufs Bf;
-    const size_t bytes = energy_carve(M, d_grad_r != nullptr, false, nullptr, Bf);
+    const size_t bytes = energy_carve(M, d_grad_r != nullptr, false, true, nullptr, Bf);
     StreamBuffer buf;
     buf.st = st;
     HIPE(hipMallocAsync((void **)&buf.p, bytes, st));
-    energy_carve(M, d_grad_r != nullptr, false, buf.p, Bf);
+    energy_carve(M, d_grad_r != nullptr, false, true, buf.p, Bf);
     return energy_adjoint_run(M, d_r, d_theta, d_E, d_grad_r, d_grad_theta, Bf, st);
 }
 
@@ -695,14 +699,14 @@ int chromo_energy_adjoint_noalloc(const chromo_energy_model *model, const double
     if (M.twist) return eset(-1, "the energy adjoint has no twist term (SSTWLC)");
     if (!d_r || !d_theta || !d_E) return eset(-1, "NULL argument");
     EnergyBufs Bf;
-    const size_t bytes = energy_carve(M, d_grad_r != nullptr, false, nullptr, Bf);
+    const size_t bytes = energy_carve(M, d_grad_r != nullptr, false, true, nullptr, Bf);
     if (bytes > NOALLOC_BYTES)
         return eset(-1, "the evaluation needs " + std::to_string(bytes) + " bytes of scratch; the allocation-free "
                         "entry point holds " + std::to_string(NOALLOC_BYTES) + " (use chromo_energy_adjoint)");
     char *base = nullptr;
     rc = noalloc_scratch(&base);
     if (rc) return rc;
-    energy_carve(M, d_grad_r != nullptr, false, base, Bf);
+    energy_carve(M, d_grad_r != nullptr, false, true, base, Bf);
     return energy_adjoint_run(M, d_r, d_theta, d_E, d_grad_r, d_grad_theta, Bf, (hipStream_t)stream);
 }
 
@@ -717,11 +721,11 @@ int chromo_energy_reference(const chromo_energy_model *model, const double *d_po
     hipStream_t st = (hipStream_t)stream;
     const int B = M.B, Bp1 = B + 1;
     EnergyBufs Bf;
-    const size_t bytes = energy_carve(M, false, d_density_out != nullptr, nullptr, Bf);
+    const size_t bytes = energy_carve(M, false, d_density_out != nullptr, false, nullptr, Bf);
     StreamBuffer buf;
     buf.st = st;
     HIPE(hipMallocAsync((void **)&buf.p, bytes, st));
-    energy_carve(M, false, d_density_out != nullptr, buf.p, Bf);
+    energy_carve(M, false, d_density_out != nullptr, false, buf.p, Bf);
     double *rho = d_density_out ? d_density_out : Bf.rho;
     const int gv = Bf.gv, gb = Bf.gb;
     // UniformDensityField.update_all_densities: bead-order scatter without the 1e-18 clean-up

@@ -301,8 +301,8 @@ int chromo_energy_adjoint(const chromo_energy_model *model, const double *d_r,
  * adjoint, the block partials) lives in one file-scope device array of 32 MiB
  * per GPU, held until the process ends; it calls no hipMalloc / hipFree.  An
  * evaluation that needs more is rejected (-1, the bytes it needs in the message):
- * 40 B per bead plus 12 + 8 (B+1) B per voxel (12 + 16 (B+1) with gradients),
- * e.g. 20,000 beads and 48^3 voxels with B = 3 take 9 MiB.  Every call on one GPU shares that
+ * 72 B per bead plus 12 + 8 (B+1) B per voxel (12 + 16 (B+1) with gradients),
+ * e.g. 20,000 beads and 48^3 voxels with B = 3 take 10 MiB.  Every call on one GPU shares that
  * array and it keeps the last call's contents: calls that overlap on one GPU
  * (two streams, two host threads) race. */
 int chromo_energy_adjoint_noalloc(const chromo_energy_model *model, const double *d_r,
