@@ -35,7 +35,7 @@ ERROR_TEXT = {
     1: "bead selection window larger than the polymer (ValueError in bead_selection)",
     2: "move touches more beads than the replica's scratch arena holds (seg_cap)",
     3: "tangent rotation asked for more distinct beads than the polymer has",
-    4: "binder count outside 1..4",
+    4: "binder count outside 1..4, or not in the launch's binder_mask",
     5: "a rejection-sampling or probe loop exceeded its iteration bound (amplitude below 1?)",
     6: "internal move-arena index check failed",
     7: "a twisting chain (SSTWLC) runs with one wavefront per replica (waves=1)",

@@ -37,7 +37,7 @@ enum : int {
     ERR_WINDOW = 1,       // bead selection window larger than the chain (ValueError)
     ERR_SEGCAP = 2,       // move larger than the scratch arena
     ERR_TANGENT_K = 3,    // more distinct beads requested than the chain has
-    ERR_NBINDERS = 4,     // binding move on a chain without binders
+    ERR_NBINDERS = 4,     // binder count outside 1..4 or not in the launch's binder_mask
     ERR_SPIN = 5,         // a rejection or probe loop ran past SPIN_LIMIT iterations
     ERR_ARENA = 6,        // a move-arena index check failed (internal error)
     ERR_TWIST_WAVES = 7,  // a twisting chain in the speculative (two-wavefront) kernel

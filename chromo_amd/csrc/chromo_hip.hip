@@ -76,6 +76,60 @@ __global__ void __launch_bounds__(64) k_mc_run(const chromo_replica *__restrict_
     counters_store(R, K, err);
 }
 
+// k_mc_run for the replicas with NB binder types and twist flag TW only; the others
+// are left to the launch of their own kind.  One instantiation per kernel, so
+// that its registers are allocated for that one alone (the all-kinds kernel above
+// spills for the largest of its eight).
+template <int NB, bool TW>
+__global__ void __launch_bounds__(64) k_mc_run_nb(const chromo_replica *__restrict__ reps, int n_reps,
+                                                  int64_t steps)
+{
+    const int r = blockIdx.x;
+    if (r >= n_reps) return;
+    __shared__ chromo_replica Rs;
+    load_record(&Rs, &reps[r]);
+    wave_sync();
+    const chromo_replica *R = &Rs;
+    if (R->B != NB || (R->twist != 0) != TW) return;
+    if (R->counters[CHROMO_CNT_ERROR] != 0) return;
+    constexpr int Bp1 = NB + 1;
+    const SegLds G = seg_lds(lds_dyn, Bp1);
+    seg_clear(G);
+    uint32_t *mt_lds = (uint32_t *)(lds_dyn + wide_lds_bytes(Bp1));
+    const bool need_mt = any_binding(R);
+    WaveRng rng;
+    rng_load(R, rng, mt_lds, need_mt);
+    Counters K;
+    counters_load(R, K);
+    const int nc = R->n_ctrl < MAX_CTRL ? R->n_ctrl : MAX_CTRL;
+    __shared__ chromo_ctrl sctrl[MAX_CTRL];
+    if (lane_id() == 0)
+        for (int i = 0; i < nc; ++i) sctrl[i] = R->ctrl[i];
+    __syncthreads();
+    int err = 0;
+    run_steps<NB, TW>(R, rng, K, sctrl, nc, steps, err, G, lds_dyn);
+    rng_store(R, rng, need_mt);
+    if (lane_id() == 0)
+        for (int i = 0; i < nc; ++i) R->ctrl[i] = sctrl[i];
+    counters_store(R, K, err);
+}
+
+template <int NB, bool TW>
+hipError_t launch_run_nb(const chromo_replica *d_reps, int n_reps, int64_t steps, hipStream_t st)
+{
+    // the attribute is set once per process: setting it costs far more than a launch
+    static bool attr_set = false;
+    const size_t bytes = mc_lds_bytes(NB + 1);
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_mc_run_nb<NB, TW>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((k_mc_run_nb<NB, TW>), dim3(n_reps), dim3(WAVE), bytes, st, d_reps, n_reps, steps);
+    return hipGetLastError();
+}
+
 // One mc_step per launch (the plain path / mc_step API).
 __global__ void __launch_bounds__(64) k_mc_step(const chromo_replica *__restrict__ reps, int rep, int ci,
                                                 int max_bp1)
@@ -130,8 +184,10 @@ __global__ void k_check_binders(const chromo_replica *reps, int n_reps, int mask
     if (r >= n_reps) return;
     const chromo_replica *R = &reps[r];
     const int B = R->B;
-    if ((B < 1 || B > CHROMO_MAX_BINDERS || !(mask & (1 << B))) && R->counters[CHROMO_CNT_ERROR] == 0)
-        R->counters[CHROMO_CNT_ERROR] = ERR_NBINDERS;
+    // a twisting chain needs its twist bit: the one-wavefront launch runs it in the
+    // twisting kernel of its binder count
+    const bool bad = B < 1 || B > CHROMO_MAX_BINDERS || !(mask & (1 << B)) || (R->twist && !(mask & (1 << (8 + B))));
+    if (bad && R->counters[CHROMO_CNT_ERROR] == 0) R->counters[CHROMO_CNT_ERROR] = ERR_NBINDERS;
 }
 
 __global__ void k_update_amplitudes(const chromo_replica *reps, int rep, int ci)
@@ -715,13 +771,13 @@ int chromo_mc_run_tuned(const chromo_replica *d_reps, int32_t n_reps, int64_t nu
     int rc = check_reps(d_reps, n_reps);
     if (rc) return rc;
     if (num_steps < 0) return set_err(-1, "num_steps < 0");
-    if ((binder_mask & 0x1e) == 0 || (binder_mask & ~0x1e) != 0)
-        return set_err(-1, "binder_mask must set only bits 1..4 (binder counts present)");
+    if ((binder_mask & 0x1e) == 0 || (binder_mask & ~0x1e1e) != 0 || ((binder_mask >> 8) & ~binder_mask & 0x1e))
+        return set_err(-1, "binder_mask must set only bits 1..4 (binder counts present) and 9..12 "
+                           "(binder counts with twisting chains)");
     const int W = waves_per_replica;
     if (W == 4 || W == 8)
-        return set_err(-1, "waves_per_replica = 4 and 8 are disabled: those builds spill heavily "
-                           "and gave a GPU fault (8), a hang and wrong voxel counts (4) whose "
-                           "cause is not found yet");
+        return set_err(-1, "waves_per_replica = 4 and 8 are not built: the speculative round has "
+                           "one wavefront on each side of its flags");
     if (W != 1 && W != 2) return set_err(-1, "waves_per_replica must be 1 or 2");
     rc = ensure_lds();
     if (rc) return rc;
@@ -734,12 +790,20 @@ int chromo_mc_run_tuned(const chromo_replica *d_reps, int32_t n_reps, int64_t nu
                        binder_mask);
     HIPC(launch_err());
     if (W == 1) {
-        hipLaunchKernelGGL(k_mc_run, dim3(n_reps), dim3(WAVE), mc_lds_bytes(bp1), st, d_reps, n_reps,
-                           num_steps, bp1);
-        HIPC(launch_err());
+        // one kernel per binder count (and twist flag) present
+        hipError_t e = hipSuccess;
+        if ((binder_mask & 2) && e == hipSuccess) e = launch_run_nb<1, false>(d_reps, n_reps, num_steps, st);
+        if ((binder_mask & 4) && e == hipSuccess) e = launch_run_nb<2, false>(d_reps, n_reps, num_steps, st);
+        if ((binder_mask & 8) && e == hipSuccess) e = launch_run_nb<3, false>(d_reps, n_reps, num_steps, st);
+        if ((binder_mask & 16) && e == hipSuccess) e = launch_run_nb<4, false>(d_reps, n_reps, num_steps, st);
+        if ((binder_mask & 0x200) && e == hipSuccess) e = launch_run_nb<1, true>(d_reps, n_reps, num_steps, st);
+        if ((binder_mask & 0x400) && e == hipSuccess) e = launch_run_nb<2, true>(d_reps, n_reps, num_steps, st);
+        if ((binder_mask & 0x800) && e == hipSuccess) e = launch_run_nb<3, true>(d_reps, n_reps, num_steps, st);
+        if ((binder_mask & 0x1000) && e == hipSuccess) e = launch_run_nb<4, true>(d_reps, n_reps, num_steps, st);
+        HIPC(e);
         return 0;
     }
-    hipError_t e = chromo::spec_launch_w2(d_reps, n_reps, num_steps, binder_mask, st);
+    hipError_t e = chromo::spec_launch_w2(d_reps, n_reps, num_steps, binder_mask & 0x1e, st);
     if (e == hipErrorInvalidValue)
         return set_err(-1, "LDS of this launch shape exceeds the device limit");
     HIPC(e);

@@ -199,7 +199,9 @@ class DeviceReplicas:
             if f.field_on and max(int(f.nx), int(f.ny), int(f.nz)) > 32767:
                 raise NotImplementedError("more than 32767 voxels along one axis")
         self.Ns, self.Bs = Ns, Bs
-        self.binder_mask = sum(1 << b for b in set(Bs))
+        # bit B: B binder types present; bit 8 + B: twisting chains among them
+        self.binder_mask = (sum(1 << b for b in set(Bs)) |
+                            sum(1 << (8 + c.B) for c in chains if c.twist))
         nctrl = [len(c) for c in ctrls]
         if max(nctrl) > self.CTRL_SLOTS:
             raise NotImplementedError(f"at most {self.CTRL_SLOTS} controllers per replica")

@@ -107,7 +107,7 @@ enum chromo_replica_error {
     CHROMO_ERR_WINDOW = 1,      /* bead window larger than the chain (the reference's ValueError) */
     CHROMO_ERR_SEGCAP = 2,      /* move longer than the replica's scratch arena (bead_cap / seg_cap) */
     CHROMO_ERR_TANGENT_K = 3,   /* more distinct tangent beads than the chain has */
-    CHROMO_ERR_NBINDERS = 4,    /* binder count outside 1..4 */
+    CHROMO_ERR_NBINDERS = 4,    /* binder count outside 1..4, or not in the launch's binder_mask */
     CHROMO_ERR_SPIN = 5,        /* a rejection or probe loop passed its bound (amplitude below one bead:
                                    the reference loops for ever) */
     CHROMO_ERR_ARENA = 6,       /* a scratch-arena index check failed */
@@ -208,12 +208,13 @@ int chromo_density_init(const chromo_replica *d_reps, int32_t n_reps, void *stre
 int chromo_mc_run(const chromo_replica *d_reps, int32_t n_reps, int64_t num_steps,
                   void *stream);
 /* chromo_mc_run with an explicit launch shape.  waves_per_replica = 1 runs each
- * chain on one wavefront, one proposal at a time; 2, 4 or 8 wavefronts evaluate
- * that many proposals of a controller block at once and resolve them in order,
+ * chain on one wavefront, one proposal at a time; 2 wavefronts evaluate two
+ * proposals of a controller block at once and resolve them in order,
  * re-evaluating any proposal an earlier accepted one in the same round touched,
  * so the chains (decisions, states, densities, RNG streams) are the same for
  * every waves_per_replica.  binder_mask has bit B set for each binder count B
- * (1..4) among the replicas; a replica whose B is not in it fails with the
+ * (1..4) among the replicas, and bit 8 + B for each B with twisting chains
+ * (SSTWLC, one wavefront only); a replica whose B is not in it fails with the
  * binder-count error.  No reference counterpart: it is a launch-shape variant
  * of mc_sim (chromo/mc/mc_sim.pyx:26-103). */
 int chromo_mc_run_tuned(const chromo_replica *d_reps, int32_t n_reps, int64_t num_steps,
