@@ -208,7 +208,7 @@ __device__ __forceinline__ void arena_sync()
 }
 
 template <int KIND, int NB, bool TW = false>
-__device__ __forceinline__ void move_body(const chromo_replica *R, double u, Proposal &P, MoveOut &out)
+__device__ __forceinline__ void move_body(const chromo_replica *R, double u, const Proposal &P, MoveOut &out)
 {
     const int lane = lane_id();
     const int64_t N = R->N;
@@ -629,17 +629,17 @@ __device__ __forceinline__ void load_pos3(const double *Pp_, int64_t N, int64_t 
     v[2] = Pp[2 * N + b];
 }
 
-// G: the caller's SEG_MAX evaluator block.  wide: the start of an LDS region that
-// holds a SEG_WIDE block (nullptr if none) and overlaps nstd SEG_MAX blocks, whose
-// hashes are emptied again after it is used.
-template <int NB, bool TW = false>
-__device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rng, chromo_ctrl &C, MoveOut &out,
-                                            const SegLds &G, char *wide, int nstd)
+// mc_step's proposal: the move's draws and their arithmetic, the capacity checks
+// and the Metropolis draw u (mc_sim.pyx:106-171).  Returns true when the
+// proposal goes on to its evaluation (n > 0, no error); out then holds n, ind0,
+// indf, and out.err any error.
+template <int NB>
+__device__ __forceinline__ bool mc_propose(const chromo_replica *R, WaveRng &rng, chromo_ctrl &C, MoveOut &out,
+                                           Proposal &P, double &u)
 {
     const int lane = lane_id();
     const int64_t N = R->N;
     const double *Pp = R->pos;
-    Proposal P;
     P.move = C.move;
     P.binder = 0;
     P.mypick = 0;
@@ -845,7 +845,7 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
     out.n = n;
     out.ind0 = ind0;
     out.indf = indf;
-    if (n == 0) return;   // mc_sim.pyx:149-151: no dE, no Metropolis draw
+    if (n == 0) return false;   // mc_sim.pyx:149-151: no dE, no Metropolis draw
     if (P.move == CHROMO_TANGENT_ROTATION) {
         // report the first picked bead
         out.ind0 = readlane_i64(P.mypick, 0);
@@ -858,9 +858,37 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
             goto fail;
         }
     }
+    // the Metropolis draw follows the proposal's draws (mc_sim.pyx:171)
+    u = rnd_unit(rng);
+    return true;
+fail:
+    out.err = err;
+    out.n = 0;
+    return false;
+}
+
+// AcceptanceTracker.update_acceptance_rate (mc_stat.py:190-208)
+__device__ __forceinline__ void track_acceptance(chromo_ctrl &C, bool accepted)
+{
+    if (accepted) {
+        C.num_success += 1;
+        C.rate = (C.alpha * 1.0) + (1 - C.alpha) * C.rate;
+    } else {
+        C.rate = (C.alpha * 0.0) + (1 - C.alpha) * C.rate;
+    }
+}
+
+// mc_step's evaluation of proposal P with Metropolis draw u: the evaluator that
+// holds it, the commit of an accepted move, the acceptance tracker.  G: the
+// caller's SEG_MAX evaluator block.  wide: the start of an LDS region that holds a
+// SEG_WIDE block (nullptr if none) and overlaps nstd SEG_MAX blocks, whose hashes
+// are emptied again after it is used.
+template <int NB, bool TW = false>
+__device__ __forceinline__ void mc_evaluate(const chromo_replica *R, chromo_ctrl &C, MoveOut &out, const Proposal &P,
+                                            double u, const SegLds &G, char *wide, int nstd)
+{
+    const int64_t n = P.n;
     {
-        // the Metropolis draw follows the proposal's draws (mc_sim.pyx:171)
-        const double u = rnd_unit(rng);
         SegRes sr;
         sr.overflow = 1;
         if (n <= SEG_MAX) {
@@ -906,8 +934,9 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
         if (sr.overflow) {
             const bool uses_field = R->field_on && P.move != CHROMO_TANGENT_ROTATION;
             if (n > R->bead_cap || (uses_field && n > R->seg_cap)) {
-                err = ERR_SEGCAP;
-                goto fail;
+                out.err = ERR_SEGCAP;
+                out.n = 0;
+                return;
             }
             move_body<1, NB, TW>(R, u, P, out);
         }
@@ -916,17 +945,17 @@ __device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rn
         out.n = 0;
         return;
     }
-    // AcceptanceTracker.update_acceptance_rate (mc_stat.py:190-208)
-    if (out.accepted) {
-        C.num_success += 1;
-        C.rate = (C.alpha * 1.0) + (1 - C.alpha) * C.rate;
-    } else {
-        C.rate = (C.alpha * 0.0) + (1 - C.alpha) * C.rate;
-    }
-    return;
-fail:
-    out.err = err;
-    out.n = 0;
+    track_acceptance(C, out.accepted);
+}
+
+// mc_step (mc_sim.pyx:106-182) on one wavefront: mc_propose, then mc_evaluate.
+template <int NB, bool TW = false>
+__device__ __forceinline__ void mc_step_dev(const chromo_replica *R, WaveRng &rng, chromo_ctrl &C, MoveOut &out,
+                                            const SegLds &G, char *wide, int nstd)
+{
+    Proposal P;
+    double u = 0;
+    if (mc_propose<NB>(R, rng, C, out, P, u)) mc_evaluate<NB, TW>(R, C, out, P, u, G, wide, nstd);
 }
 
 // SimpleControl.update_amplitudes (mc_controller.py:126-213)

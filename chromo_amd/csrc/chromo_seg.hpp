@@ -176,6 +176,20 @@ __device__ __forceinline__ void fold_entry(const FoldEntry &f, double *acc, bool
     }
 }
 
+// The fold of one voxel's list (cnt entries from off) into acc, in the list's
+// order (the reference's), two entries requested at a time.  Whole wave.
+template <int NB>
+__device__ __forceinline__ void fold_list(const FoldEntry *list, int off, int cnt, double *acc, bool thr)
+{
+    for (int k = 0; wave_any(k < cnt); k += 2) {
+        FoldEntry f0 = {0., 0, 0}, f1 = {0., 0, 0};
+        if (k < cnt) f0 = list[off + k];
+        if (k + 1 < cnt) f1 = list[off + k + 1];
+        if (k < cnt) fold_entry<NB>(f0, acc, thr);
+        if (k + 1 < cnt) fold_entry<NB>(f1, acc, thr);
+    }
+}
+
 // Contribution e of position p goes to entry off(q) + (positions of q's mask
 // below p) of its voxel q's fold list (hmin maps its hash slot to q, ucrd.y's
 // upper half holds off(q)).  The arrays are distinct LDS regions: with no
@@ -184,12 +198,14 @@ template <int MW, int ROUNDS>
 __device__ __forceinline__ void seg_build_lists(int E, int rounds, const int32_t *__restrict__ eslot,
                                                 const int32_t *__restrict__ hmin, const uint64_t *__restrict__ umask,
                                                 const int2 *__restrict__ ucrd, const double *__restrict__ c0,
-                                                const int4 *__restrict__ prec, FoldEntry *__restrict__ list)
+                                                const int4 *__restrict__ prec, FoldEntry *__restrict__ list,
+                                                int r0 = 0, int rstep = 1)
 {
     const int lane = lane_id();
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) {
         if (r >= rounds) break;
+        if ((r - r0) % rstep != 0) continue;
         const int e = lane + WAVE * r;
         if (e < E) {
             const int p = e >> 3;
@@ -369,11 +385,33 @@ __device__ __forceinline__ void seg_r3(const SegBeads<NB> &B, int64_t x, int64_t
 // with the uniform draw u (mc_sim.pyx:152-182).  Writes nothing outside S and K.
 // Called by one whole wavefront; S's hash is empty on entry and on return.  B
 // holds what seg_load fetched for this move.
-template <int NB, int SMAX = SEG_MAX, bool TW = false>
+// A barrier of the evaluator's phases: its wavefront, or with COOP both
+// wavefronts of the workgroup.
+template <bool COOP>
+__device__ __forceinline__ void seg_sync()
+{
+    if constexpr (COOP) __syncthreads();
+    else wave_sync();
+}
+
+// COOP: the two wavefronts of the workgroup evaluate one proposal together (a
+// crank-shaft, slide or end-pivot move with the field on, its beads and the
+// SEG_MAX-exceeding part of a sequential round).  Wavefront 1 takes the trial
+// positions' voxel floors, every other round of contributions (hash, numbering,
+// fold lists) and the odd voxel slots' folds, which it hands to wavefront 0
+// through xbuf (LDS, COOP_XBUF_BYTES); everything whose order shapes a result
+// (numbering, dealing, the energy terms, the reductions, Metropolis) runs on
+// wavefront 0 in the order of the one-wavefront evaluation, so both give the same
+// numbers.  res and K are wavefront 0's.
+constexpr int COOP_XBUF_BYTES = 256 + 2 * 64 * 5 * 8;
+
+template <int NB, int SMAX = SEG_MAX, bool TW = false, bool COOP = false>
 __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const Proposal &P, double u,
                                                 const SegLds &S, SegRes &res, SegKeep<NB> &K,
-                                                const SegBeads<NB> &B)
+                                                const SegBeads<NB> &B, char *xbuf = nullptr)
 {
+    static_assert(!COOP || (SMAX == SEG_WIDE && !TW), "the cooperative evaluation is the 64-bead one");
+    const int wv = COOP ? wave_id() : 0;
     using C = SegCap<SMAX>;
     constexpr int MW = C::MW;
     constexpr int Bp1 = NB + 1;
@@ -461,6 +499,7 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
         if (act) {
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
+                if (COOP && k != wv) continue;
                 double pt[3];
                 for (int j = 0; j < 3; ++j) pt[j] = (k == 0 || state_change) ? o[j] : w[j];
                 int64_t ix[3];
@@ -479,7 +518,7 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
             }
         }
         count_ties(R, ties);
-        wave_sync();
+        seg_sync<COOP>();
 
         // 1. unique voxels: hash the contributions in order, keep each voxel's first
         const int E = (int)(16 * n);
@@ -487,6 +526,7 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
 #pragma unroll
         for (int r = 0; r < C::ROUNDS; ++r) {
             if (r >= rounds) break;
+            if (COOP && (r & 1) != wv) continue;
             const int e = lane + WAVE * r;
             if (e < E) {
                 const int4 pr = S.prec[e >> 3];
@@ -511,11 +551,28 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
                 S.eslot[e] = (int)h;
             }
         }
-        wave_sync();
+        seg_sync<COOP>();
+        int32_t *rcnt = (int32_t *)xbuf;   // COOP: first occurrences per round
+        if constexpr (COOP) {
+#pragma unroll
+            for (int r = 0; r < C::ROUNDS; ++r) {
+                if (r >= rounds) break;
+                if ((r & 1) != wv) continue;
+                const int e = lane + WAVE * r;
+                const int h = e < E ? S.eslot[e] : 0;
+                const int c = __popcll(__ballot(e < E && S.hmin[h] == e));
+                if (lane == 0) rcnt[r] = c;
+            }
+            __syncthreads();
+        }
         int total = 0;
 #pragma unroll
         for (int r = 0; r < C::ROUNDS; ++r) {
             if (r >= rounds) break;
+            if (COOP && (r & 1) != wv) {
+                total += rcnt[r];
+                continue;
+            }
             const int e = lane + WAVE * r;
             const int h = e < E ? S.eslot[e] : 0;
             const bool first = e < E && S.hmin[h] == e;
@@ -536,7 +593,7 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
             total += cnt;
         }
         U = total;
-        wave_sync();
+        seg_sync<COOP>();
         // Unless a grid axis has one voxel (a position's corners can then share a
         // voxel), each voxel's contributions are laid out below as one list, in
         // fold order, in the position-mask area of the hash; its hmin slots map
@@ -544,7 +601,7 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
         const bool degen = nx == 1 || ny == 1 || nz == 1;
         const bool listed = !degen && U <= SEG_UMAX;
         // release the hash (every key is some voxel's first occurrence)
-        for (int q = lane; q < U && q < SEG_UMAX; q += WAVE) {
+        for (int q = lane; q < U && q < SEG_UMAX && wv == 0; q += WAVE) {
             const int h = S.uaux[q];
             S.hkey[h] = HKEY_EMPTY;
             if (!listed) {
@@ -555,7 +612,7 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
         }
         if (U > SEG_UMAX) {
             // too many voxels for the registers: the global-arena path evaluates it
-            for (int e = lane; e < E; e += WAVE) {
+            for (int e = lane; e < E && wv == 0; e += WAVE) {
                 const int h = S.eslot[e];
                 S.hkey[h] = HKEY_EMPTY;
                 S.hmin[h] = HMIN_EMPTY;
@@ -568,7 +625,7 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
             return;
         }
         wave_sync();
-        if (U > WAVE) {
+        if (U > WAVE && wv == 0) {
             // deal the voxels out longest fold first (fold_class), so that each
             // round of the fold loop below holds voxels of similar length
             int vv[SEG_VB], hh[SEG_VB], cls[SEG_VB], rank[SEG_VB];
@@ -615,7 +672,7 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
 #pragma unroll
         for (int j = 0; j < NB * NB; ++j) SX[j] = 0;
         FoldEntry *list = (FoldEntry *)S.hpos;
-        if (listed) {
+        if (listed && wv == 0) {
             // list offsets: voxels in (j, lane) order, each as long as its position mask;
             // the offset rides in the upper half of ucrd.y (z < 2^15)
             int base = 0;
@@ -635,9 +692,39 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
                     S.ucrd[q].y |= off << 16;
                 }
             }
-            wave_sync();
-            seg_build_lists<MW, C::ROUNDS>(E, rounds, S.eslot, S.hmin, S.umask, S.ucrd, S.c0, S.prec, list);
-            wave_sync();
+        }
+        seg_sync<COOP>();
+        if (listed) {
+            seg_build_lists<MW, C::ROUNDS>(E, rounds, S.eslot, S.hmin, S.umask, S.ucrd, S.c0, S.prec, list,
+                                           COOP ? wv : 0, COOP ? 2 : 1);
+            seg_sync<COOP>();
+        }
+        double *xacc = (double *)(xbuf + 256);   // COOP: [slot j / 2][m][lane] of wavefront 1
+        double accj[2][Bp1];                     // COOP: this wavefront's slots j = 2 jj + wv
+        if constexpr (COOP) {
+            if (listed) {
+#pragma unroll
+                for (int jj = 0; jj < 2; ++jj) {
+                    const int j = 2 * jj + wv;
+#pragma unroll
+                    for (int m = 0; m < Bp1; ++m) accj[jj][m] = 0;
+                    if (WAVE * j >= U) continue;
+                    const int q = lane + WAVE * j;
+                    int cnt = 0, off = 0;
+                    if (q < U) {
+#pragma unroll
+                        for (int w = 0; w < MW; ++w) cnt += __popcll(S.umask[q * MW + w]);
+                        off = S.ucrd[q].y >> 16;
+                    }
+                    fold_list<NB>(list, off, cnt, accj[jj], thr);
+                    if (wv == 1) {
+#pragma unroll
+                        for (int m = 0; m < Bp1; ++m) xacc[(jj * Bp1 + m) * WAVE + lane] = accj[jj][m];
+                    }
+                }
+            }
+            __syncthreads();
+            if (wv == 1) return;   // wavefront 0 goes on alone
         }
         if (listed) {
 #pragma unroll
@@ -662,14 +749,12 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
                     for (int w = 0; w < MW; ++w) cnt += __popcll(S.umask[q * MW + w]);
                     off = S.ucrd[q].y >> 16;
                 }
-                // the voxel's contributions in the reference's order, two entries
-                // requested at a time
-                for (int k = 0; wave_any(k < cnt); k += 2) {
-                    FoldEntry f0 = {0., 0, 0}, f1 = {0., 0, 0};
-                    if (k < cnt) f0 = list[off + k];
-                    if (k + 1 < cnt) f1 = list[off + k + 1];
-                    if (k < cnt) fold_entry<NB>(f0, acc, thr);
-                    if (k + 1 < cnt) fold_entry<NB>(f1, acc, thr);
+                if constexpr (COOP) {
+                    // this voxel's fold, by wavefront 0 (even slots) or 1 (odd slots)
+#pragma unroll
+                    for (int m = 0; m < Bp1; ++m) acc[m] = (j & 1) ? xacc[((j >> 1) * Bp1 + m) * WAVE + lane] : accj[j >> 1][m];
+                } else {
+                    fold_list<NB>(list, off, cnt, acc, thr);
                 }
                 if (own) {
                     voxel_dE_terms<NB>(R, vown, rho, acc, S2, SX, NS);
@@ -796,13 +881,13 @@ __device__ __forceinline__ void seg_eval_loaded(const chromo_replica *R, const P
 }
 
 // seg_eval_loaded with its own loads (the one-at-a-time path).
-template <int NB, int SMAX = SEG_MAX, bool TW = false>
+template <int NB, int SMAX = SEG_MAX, bool TW = false, bool COOP = false>
 __device__ __forceinline__ void seg_eval(const chromo_replica *R, const Proposal &P, double u, const SegLds &S,
-                                         SegRes &res, SegKeep<NB> &K)
+                                         SegRes &res, SegKeep<NB> &K, char *xbuf = nullptr)
 {
     SegBeads<NB> B;
     seg_load<NB>(R, P.move, P.n, P.ind0, P.indf, P.mypick, B);
-    seg_eval_loaded<NB, SMAX, TW>(R, P, u, S, res, K, B);
+    seg_eval_loaded<NB, SMAX, TW, COOP>(R, P, u, S, res, K, B, xbuf);
 }
 
 // Writes an accepted proposal: bead coordinates or states (MCAdapter.accept,

@@ -60,12 +60,17 @@ struct SpecCtl {
     int32_t stall;                            // = tag when a later wavefront gave up waiting for ins
 };
 
-// LDS exchange area of the one-at-a-time path (seq_step).
+// LDS exchange area of the one-at-a-time path (seq_step, seq_step_coop).
 struct SeqIO {
     uint32_t ck[CK_WORDS];
     int32_t mti, pad0;
     chromo_ctrl C;
     MoveOut out;
+    int32_t coop, pad1;      // seq_step_coop: both wavefronts evaluate the proposal
+    double u;                // its Metropolis draw
+    int32_t move, binder;    // its wave-uniform fields
+    int64_t n, ind0, indf;
+    double T[12];
 };
 
 struct SpecLayout {
@@ -401,33 +406,107 @@ __device__ __forceinline__ int gen_slot(const chromo_replica *R, WaveRng &rng, c
     return kind;
 }
 
-// mc_step_dev for one proposal, run by one wavefront; the RNG, the controller and
-// the result pass through LDS (io).  Inlined: a call boundary here makes the
-// callee save most of the register file on every call and the caller spill
-// around it.
+// One proposal of a sequential round (the RNG, the controller and the result
+// pass through LDS, io).  Inlined: a call boundary here makes the callee save
+// most of the register file on every call and the caller spill around it.
+// Wavefront 0 draws the proposal (mc_propose); a
+// crank-shaft, slide or end-pivot move of SEG_MAX < n <= SEG_WIDE beads with the
+// field on is then evaluated by both (seg_eval COOP, in the wide block, exchange
+// area after it), everything else by wavefront 0 alone (mc_evaluate).  The
+// numbers are those of mc_step_dev.  Both wavefronts call it.
 template <int NB, int W>
-__device__ __forceinline__ void seq_step(const chromo_replica *R, SeqIO *io, uint32_t *mt, const SegLds &G,
-                                         char *wide)
+__device__ __forceinline__ void seq_step_coop(const chromo_replica *R, SeqIO *io, uint32_t *mt, const SegLds &G,
+                                              char *wide)
 {
+    static_assert(W == 2, "two wavefronts");
+    const int wv = wave_id(), lane = lane_id();
     WaveRng rng;
-    rng.mode = uni_i32(R->rng_mode);
-    rng.k0 = uni_u32(R->rng[CHROMO_RNG_PHILOX_KEY]);
-    rng.k1 = uni_u32(R->rng[CHROMO_RNG_PHILOX_KEY + 1]);
-    rng.pbase = PHILOX_NONE;
-    rng.pidx = 0;
-    rng.mt = mt;      // the wavefront's LDS batch: rng_restore reuses it when it can
-    rng_restore(rng, io->ck);
-    rng.mti = uni_i32(io->mti);
-    chromo_ctrl C = io->C;
+    chromo_ctrl C;
     MoveOut o;
-    mc_step_dev<NB>(R, rng, C, o, G, wide, W);
-    rng_save(rng, io->ck);
-    if (lane_id() == 0) {
-        io->mti = rng.mti;
-        io->C = C;
-        io->out = o;
+    Proposal P;
+    double u = 0;
+    bool go = false;
+    const SegLds GW = seg_lds<SEG_WIDE>(wide, NB + 1);
+    if (wv == 0) {
+        rng.mode = uni_i32(R->rng_mode);
+        rng.k0 = uni_u32(R->rng[CHROMO_RNG_PHILOX_KEY]);
+        rng.k1 = uni_u32(R->rng[CHROMO_RNG_PHILOX_KEY + 1]);
+        rng.pbase = PHILOX_NONE;
+        rng.pidx = 0;
+        rng.mt = mt;
+        rng_restore(rng, io->ck);
+        rng.mti = uni_i32(io->mti);
+        C = io->C;
+        o.err = 0;
+        o.n = 0;
+        go = mc_propose<NB>(R, rng, C, o, P, u);
+        const bool coop = go && R->field_on && P.n > SEG_MAX && P.n <= SEG_WIDE &&
+                          (P.move == CHROMO_CRANK_SHAFT || P.move == CHROMO_SLIDE || P.move == CHROMO_END_PIVOT);
+        if (coop) seg_clear<SEG_WIDE>(GW);
+        if (lane == 0) {
+            io->coop = coop;
+            io->u = u;
+            io->move = P.move;
+            io->binder = P.binder;
+            io->n = P.n;
+            io->ind0 = P.ind0;
+            io->indf = P.indf;
+            for (int q = 0; q < 12; ++q) io->T[q] = P.T[q];
+        }
     }
-    wave_sync();
+    __syncthreads();
+    if (io->coop) {
+        if (wv != 0) {
+            P.move = io->move;
+            P.binder = io->binder;
+            P.n = io->n;
+            P.ind0 = io->ind0;
+            P.indf = io->indf;
+            for (int q = 0; q < 12; ++q) P.T[q] = io->T[q];
+            P.mypick = 0;
+            P.mynew = 0;
+            for (int q = 0; q < 9; ++q) P.myT[q] = 0;
+            u = io->u;
+        }
+        SegKeep<NB> K;
+        SegRes sr;
+        seg_eval<NB, SEG_WIDE, false, true>(R, P, u, GW, sr, K, wide + seg_layout<SEG_WIDE>(NB + 1).total);
+        if (wv == 0) {
+            // as mc_evaluate after its SEG_WIDE evaluation
+            if (!sr.overflow) {
+                if (sr.accepted) seg_commit<NB>(R, P.move, P.n, P.binder, P.mynew, K, GW, sr.U);
+                wave_sync_global();
+            }
+            for (int b = 0; b < W; ++b)
+                seg_clear<SEG_MAX>(seg_lds<SEG_MAX>(wide + (size_t)b * seg_layout<SEG_MAX>(NB + 1).total, NB + 1));
+            if (!sr.overflow) {
+                o.dEp = sr.dEp;
+                o.dEf = sr.dEf;
+                o.u = u;
+                o.expdE = sr.expdE;
+                o.accepted = sr.accepted;
+                o.nbins = sr.U;
+            } else if (P.n > R->bead_cap || P.n > R->seg_cap) {
+                o.err = ERR_SEGCAP;
+                o.n = 0;
+            } else {
+                move_body<1, NB>(R, u, P, o);
+            }
+            if (o.err) o.n = 0;
+            else track_acceptance(C, o.accepted);
+        }
+    } else if (wv == 0 && go) {
+        mc_evaluate<NB>(R, C, o, P, u, G, wide, W);
+    }
+    if (wv == 0) {
+        rng_save(rng, io->ck);
+        if (lane == 0) {
+            io->mti = rng.mti;
+            io->C = C;
+            io->out = o;
+        }
+        wave_sync();
+    }
 }
 
 // The proposal of slot S from its stored draws: the arithmetic of mc_step_dev's
@@ -807,9 +886,12 @@ __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng
                     io->C = C;
                 }
                 wave_sync();
-                // the other wavefronts wait at the barrier: wavefront 0 may use all
-                // W evaluator blocks as one wide block
-                seq_step<NB, W>(R, io, L.mt, L.first, L.wide);
+            }
+            // wavefront 0 may use all W evaluator blocks as one wide block; a
+            // 64-bead move both evaluate
+            seq_step_coop<NB, W>(R, L.seq, L.mt, L.first, L.wide);
+            if (wv == 0) {
+                SeqIO *io = L.seq;
                 rng_restore(rng, io->ck);
                 rng.mti = uni_i32(io->mti);
                 if (lane == 0) C = io->C;
