@@ -35,8 +35,13 @@ constexpr int BIG_HASH = 2048;              // hash slots
 constexpr int BIG_HASH_SHIFT = 21;          // 32 - log2(BIG_HASH)
 static_assert(BIG_POS == 64, "one mask word per chunk");
 
+// Exchange words of the two-wavefront evaluation: per hash round the number of
+// first contributions and of newly numbered voxels, and the hash-full flag.
+constexpr int BIG_ROUNDS = BIG_E / WAVE;
+constexpr int BIG_XCH_WORDS = 2 * BIG_ROUNDS + 2;
+
 struct BigLayout {
-    size_t c0, pw, prec, eslot, tlist, hkey, hq, hpos, uvox, uacc, total;
+    size_t c0, pw, prec, eslot, tlist, hkey, hq, hpos, xch, uvox, uacc, total;
     int ucap;   // unique voxels the block holds (0: too small to use)
 };
 
@@ -55,6 +60,7 @@ __host__ __device__ inline BigLayout big_layout(int Bp1, size_t bytes)
     L.hpos = o;  o += sizeof(uint64_t) * BIG_HASH;       // chunk positions touching the voxel
     L.hkey = o;  o += sizeof(int32_t) * BIG_HASH;        // voxel
     L.hq = o;    o += sizeof(int32_t) * BIG_HASH;        // its number, -1 before it has one
+    L.xch = o;   o += sizeof(int32_t) * BIG_XCH_WORDS;   // two-wavefront exchange
     const size_t per = sizeof(int32_t) + sizeof(double) * (size_t)Bp1;
     long avail = (long)bytes - (long)o - 16;
     long cap = avail > 0 ? avail / (long)per : 0;
@@ -69,7 +75,7 @@ __host__ __device__ inline BigLayout big_layout(int Bp1, size_t bytes)
 struct BigLds {
     double *c0, *pw, *uacc;
     int4 *prec;
-    int32_t *eslot, *tlist, *hkey, *hq, *uvox;
+    int32_t *eslot, *tlist, *hkey, *hq, *uvox, *xch;
     uint64_t *hpos;
     int ucap;
 };
@@ -86,6 +92,7 @@ __device__ __forceinline__ BigLds big_lds(char *base, int Bp1, size_t bytes)
     S.hpos = (uint64_t *)(base + L.hpos);
     S.hkey = (int32_t *)(base + L.hkey);
     S.hq = (int32_t *)(base + L.hq);
+    S.xch = (int32_t *)(base + L.xch);
     S.uacc = (double *)(base + L.uacc);
     S.uvox = (int32_t *)(base + L.uvox);
     S.ucap = L.ucap;
@@ -106,17 +113,105 @@ __device__ __forceinline__ int big_corner(const int4 &pr, int l, int nx, int ny)
     return x + nx * (y + ny * z);
 }
 
+// The first-occurrence numbering of one chunk by both wavefronts (wavefront wv
+// takes the hash rounds r = wv mod 2): pass 1 finds each round's first
+// contributions and newly seen voxels and publishes their counts in S.xch; pass 2
+// places them after the earlier rounds' ones.  T (0 on entry) and U come out as
+// the one-wavefront loop leaves them.
+template <int NB>
+__device__ __forceinline__ void big_number_coop(const BigLds &S, int rounds, int E, bool degen, int nx, int ny,
+                                                int wv, int &T, int &U)
+{
+    constexpr int Bp1 = NB + 1;
+    constexpr int K = (BIG_ROUNDS + 1) / 2;
+    const int lane = lane_id();
+    bool first[K], fresh[K];
+    int hs[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int r = wv + 2 * k;
+        const int e = lane + WAVE * r;
+        first[k] = false;
+        fresh[k] = false;
+        hs[k] = 0;
+        if (r < rounds && e < E) {
+            const int p = e >> 3, l = e & 7;
+            const int h = S.eslot[e];
+            bool f = (S.hpos[h] & ((1ull << p) - 1)) == 0;   // no earlier position touches it
+            if (f && degen) {
+                // corners of one position can coincide on a one-voxel axis
+                const int4 pr = S.prec[p];
+                const int v = big_corner(pr, l, nx, ny);
+                for (int l2 = 0; l2 < l; ++l2)
+                    if (big_corner(pr, l2, nx, ny) == v) f = false;
+            }
+            first[k] = f;
+            fresh[k] = f && S.hq[h] < 0;
+            hs[k] = h;
+        }
+        if (r < rounds) {
+            const int ct = __popcll(__ballot(first[k])), cn = __popcll(__ballot(fresh[k]));
+            if (lane == 0) {
+                S.xch[r] = ct;
+                S.xch[BIG_ROUNDS + r] = cn;
+            }
+        }
+    }
+    __syncthreads();
+    int tk[K], uk[K];
+    int toff = T, uoff = U;
+#pragma unroll
+    for (int r = 0; r < BIG_ROUNDS; ++r) {
+        if (r < rounds) {
+            if ((r & 1) == wv) {
+                tk[r >> 1] = toff;
+                uk[r >> 1] = uoff;
+            }
+            toff += S.xch[r];
+            uoff += S.xch[BIG_ROUNDS + r];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int r = wv + 2 * k;
+        if (r >= rounds) continue;
+        int tcnt, ncnt;
+        const int t = tk[k] + wave_excl_count(first[k], tcnt);
+        if (first[k]) S.tlist[t] = hs[k];
+        const int q = uk[k] + wave_excl_count(fresh[k], ncnt);
+        if (fresh[k] && q < S.ucap) {
+            S.hq[hs[k]] = q;
+            S.uvox[q] = S.hkey[hs[k]];
+#pragma unroll
+            for (int m = 0; m < Bp1; ++m) S.uacc[q * Bp1 + m] = 0;
+        }
+    }
+    T = toff;
+    U = uoff;
+}
+
 // Evaluates the crank-shaft, end-pivot or slide proposal P (n >= 1 beads, any
 // length) against the committed state, like seg_eval: polymer dE
 // (SSWLC.compute_dE, polymers.pyx:1024-1082), field dE
 // (UniformDensityField.compute_dE, fields.pyx:1149-1233), Metropolis with u
 // (mc_sim.pyx:152-182).  Writes nothing outside S.  Called by one whole wavefront.
-template <int NB, bool TW = false>
+//
+// COOP: both wavefronts of the workgroup call it with the same proposal.  Per
+// chunk, wavefront k places the positions 2i + k, the hash rounds alternate
+// between them, the first-occurrence numbering is a two-pass prefix over the
+// rounds (counts through S.xch), and the touched voxels' folds alternate by
+// blocks of 64; the ordering by fold length and the energy terms stay on
+// wavefront 0, which alone holds the result (wavefront 1 returns before the
+// energy terms).  Every number is the one-wavefront evaluation's.
+template <int NB, bool TW = false, bool COOP = false>
 __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal &P, double u, const BigLds &S,
                                          SegRes &res)
 {
+    static_assert(!COOP || !TW, "the cooperative evaluation has no twist terms");
     constexpr int Bp1 = NB + 1;
     const int lane = lane_id();
+    const int wv = COOP ? wave_id() : 0;
+    constexpr int NW = COOP ? 2 : 1;
     const int64_t N = R->N;
     const GLOBAL_AS double *Pp = gmem(R->pos);
     const GLOBAL_AS int8_t *states = gmem(R->states);
@@ -132,12 +227,13 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
     res.overflow = 0;
     const VoxGrid VG = vox_grid(R);
 
-    for (int i = lane; i < BIG_HASH; i += WAVE) {
+    for (int i = lane + WAVE * wv; i < BIG_HASH; i += WAVE * NW) {
         S.hkey[i] = HKEY_EMPTY;
         S.hq[i] = -1;
         S.hpos[i] = 0;
     }
-    wave_sync();
+    if (COOP && wv == 0 && lane == 0) S.xch[2 * BIG_ROUNDS] = 0;
+    seg_sync<COOP>();
 
     double ef = 0, er = 0;
     int dbl[NB];
@@ -212,6 +308,7 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
             for (int m = 0; m < NB; ++m) sp |= ((int32_t)(uint8_t)so[m]) << (8 * m);
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
+                if (COOP && k != wv) continue;
                 double pt[3];
                 for (int j = 0; j < 3; ++j) pt[j] = k == 0 ? o[j] : w[j];
                 int64_t ix[3];
@@ -226,13 +323,13 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
             }
         }
         count_ties(R, ties);
-        wave_sync();
+        seg_sync<COOP>();
 
         // hash the chunk's contributions; each slot collects the positions touching it
         const int E = 16 * nb;
         const int rounds = (E + WAVE - 1) / WAVE;
         int full = 0;
-        for (int r = 0; r < rounds; ++r) {
+        for (int r = wv; r < rounds; r += NW) {
             const int e = lane + WAVE * r;
             if (e < E) {
                 const int p = e >> 3, l = e & 7;
@@ -257,15 +354,23 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
                 }
             }
         }
+        if constexpr (COOP) {
+            if (wave_any(full) && lane == 0) S.xch[2 * BIG_ROUNDS] = 1;
+            __syncthreads();
+            full = S.xch[2 * BIG_ROUNDS];
+        }
         if (wave_any(full)) {
             over = 1;
             break;
         }
-        wave_sync();
+        if constexpr (!COOP) wave_sync();
 
         // first contribution of each touched voxel in this chunk: the list of touched
         // slots, and numbers for the voxels not seen in earlier chunks
         int T = 0;
+        if constexpr (COOP) {
+            big_number_coop<NB>(S, rounds, E, degen, nx, ny, wv, T, U);
+        } else {
         for (int r = 0; r < rounds; ++r) {
             const int e = lane + WAVE * r;
             bool first = false;
@@ -298,12 +403,13 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
             }
             U += ncnt;
         }
+        }
         if (U > S.ucap) {
             over = 1;
             break;
         }
-        wave_sync();
-        if (T > WAVE) {
+        seg_sync<COOP>();
+        if (T > WAVE && wv == 0) {
             // touched voxels longest fold first (fold_class): each round of the
             // fold loop below then holds voxels of similar length
             constexpr int J = BIG_E / WAVE;
@@ -322,9 +428,10 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
                 if (cls[j] < 8) S.tlist[rank[j]] = hh[j];
             wave_sync();
         }
+        if (COOP && T > WAVE) __syncthreads();
 
         // fold: touched voxel t on lane t mod 64, its positions in increasing order
-        for (int t0 = 0; t0 < T; t0 += WAVE) {
+        for (int t0 = WAVE * wv; t0 < T; t0 += WAVE * NW) {
             const int t = t0 + lane;
             const bool own = t < T;
             int h = 0, q = 0, vx = -1, vy = -1, vz = -1;
@@ -373,13 +480,14 @@ __device__ __forceinline__ void big_eval(const chromo_replica *R, const Proposal
                 S.hpos[h] = 0;
             }
         }
-        wave_sync();
+        seg_sync<COOP>();
     }
     if (over) {
         res.overflow = 1;
         res.U = U;
         return;
     }
+    if (COOP && wv != 0) return;
 
     double dEp = 0;
     if (ind0 != 0) dEp += ef;

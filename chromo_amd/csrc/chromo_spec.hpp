@@ -66,7 +66,7 @@ struct SeqIO {
     int32_t mti, pad0;
     chromo_ctrl C;
     MoveOut out;
-    int32_t coop, pad1;      // seq_step_coop: both wavefronts evaluate the proposal
+    int32_t coop, pad1;      // seq_step_coop: 0 wavefront 0 alone, 1 both in seg_eval, 2 both in big_eval
     double u;                // its Metropolis draw
     int32_t move, binder;    // its wave-uniform fields
     int64_t n, ind0, indf;
@@ -410,10 +410,11 @@ __device__ __forceinline__ int gen_slot(const chromo_replica *R, WaveRng &rng, c
 // pass through LDS, io).  Inlined: a call boundary here makes the callee save
 // most of the register file on every call and the caller spill around it.
 // Wavefront 0 draws the proposal (mc_propose); a
-// crank-shaft, slide or end-pivot move of SEG_MAX < n <= SEG_WIDE beads with the
-// field on is then evaluated by both (seg_eval COOP, in the wide block, exchange
-// area after it), everything else by wavefront 0 alone (mc_evaluate).  The
-// numbers are those of mc_step_dev.  Both wavefronts call it.
+// crank-shaft, slide or end-pivot move of more than SEG_MAX beads with the field
+// on is then evaluated by both (up to SEG_WIDE beads seg_eval COOP, in the wide
+// block, exchange area after it; longer ones big_eval COOP), everything else by
+// wavefront 0 alone (mc_evaluate).  The numbers are those of mc_step_dev.  Both
+// wavefronts call it.
 template <int NB, int W>
 __device__ __forceinline__ void seq_step_coop(const chromo_replica *R, SeqIO *io, uint32_t *mt, const SegLds &G,
                                               char *wide)
@@ -427,6 +428,7 @@ __device__ __forceinline__ void seq_step_coop(const chromo_replica *R, SeqIO *io
     double u = 0;
     bool go = false;
     const SegLds GW = seg_lds<SEG_WIDE>(wide, NB + 1);
+    const BigLds GB = big_lds(wide, NB + 1, wide_lds_bytes(NB + 1));
     if (wv == 0) {
         rng.mode = uni_i32(R->rng_mode);
         rng.k0 = uni_u32(R->rng[CHROMO_RNG_PHILOX_KEY]);
@@ -440,10 +442,15 @@ __device__ __forceinline__ void seq_step_coop(const chromo_replica *R, SeqIO *io
         o.err = 0;
         o.n = 0;
         go = mc_propose<NB>(R, rng, C, o, P, u);
-        const bool coop = go && R->field_on && P.n > SEG_MAX && P.n <= SEG_WIDE &&
-                          (P.move == CHROMO_CRANK_SHAFT || P.move == CHROMO_SLIDE || P.move == CHROMO_END_PIVOT);
-        if (coop) seg_clear<SEG_WIDE>(GW);
+        // the RNG and the controller go back to LDS now, so that they are not
+        // held in registers through the evaluation
+        rng_save(rng, io->ck);
+        const bool both = go && R->field_on && wide != nullptr && P.n > SEG_MAX && big_move(P.move);
+        const int coop = !both ? 0 : P.n <= SEG_WIDE ? 1 : GB.ucap > 0 ? 2 : 0;
+        if (coop == 1) seg_clear<SEG_WIDE>(GW);
         if (lane == 0) {
+            io->mti = rng.mti;
+            io->C = C;
             io->coop = coop;
             io->u = u;
             io->move = P.move;
@@ -455,7 +462,8 @@ __device__ __forceinline__ void seq_step_coop(const chromo_replica *R, SeqIO *io
         }
     }
     __syncthreads();
-    if (io->coop) {
+    const int coop = io->coop;
+    if (coop) {
         if (wv != 0) {
             P.move = io->move;
             P.binder = io->binder;
@@ -468,15 +476,23 @@ __device__ __forceinline__ void seq_step_coop(const chromo_replica *R, SeqIO *io
             for (int q = 0; q < 9; ++q) P.myT[q] = 0;
             u = io->u;
         }
-        SegKeep<NB> K;
         SegRes sr;
-        seg_eval<NB, SEG_WIDE, false, true>(R, P, u, GW, sr, K, wide + seg_layout<SEG_WIDE>(NB + 1).total);
-        if (wv == 0) {
-            // as mc_evaluate after its SEG_WIDE evaluation
-            if (!sr.overflow) {
+        if (coop == 1) {
+            SegKeep<NB> K;
+            seg_eval<NB, SEG_WIDE, false, true>(R, P, u, GW, sr, K, wide + seg_layout<SEG_WIDE>(NB + 1).total);
+            if (wv == 0 && !sr.overflow) {
                 if (sr.accepted) seg_commit<NB>(R, P.move, P.n, P.binder, P.mynew, K, GW, sr.U);
                 wave_sync_global();
             }
+        } else {
+            big_eval<NB, false, true>(R, P, u, GB, sr);
+            if (wv == 0 && !sr.overflow) {
+                if (sr.accepted) big_commit<NB>(R, P, GB, sr.U);
+                wave_sync_global();
+            }
+        }
+        if (wv == 0) {
+            // as mc_evaluate after its SEG_WIDE or long-move evaluation
             for (int b = 0; b < W; ++b)
                 seg_clear<SEG_MAX>(seg_lds<SEG_MAX>(wide + (size_t)b * seg_layout<SEG_MAX>(NB + 1).total, NB + 1));
             if (!sr.overflow) {
@@ -493,15 +509,15 @@ __device__ __forceinline__ void seq_step_coop(const chromo_replica *R, SeqIO *io
                 move_body<1, NB>(R, u, P, o);
             }
             if (o.err) o.n = 0;
-            else track_acceptance(C, o.accepted);
+            if (lane == 0) {
+                if (!o.err) track_acceptance(io->C, o.accepted);
+                io->out = o;
+            }
+            wave_sync();
         }
-    } else if (wv == 0 && go) {
-        mc_evaluate<NB>(R, C, o, P, u, G, wide, W);
-    }
-    if (wv == 0) {
-        rng_save(rng, io->ck);
+    } else if (wv == 0) {
+        if (go) mc_evaluate<NB>(R, C, o, P, u, G, wide, W);
         if (lane == 0) {
-            io->mti = rng.mti;
             io->C = C;
             io->out = o;
         }
