@@ -316,56 +316,79 @@ __global__ void k_dens_permute(int64_t N, const int32_t *list, const BeadRec *br
 
 // The density of the energy adjoint, which needs a deterministic sum but not the
 // reference's bead-order fold: voxel v sums each of its (<= 8) sorted buckets on
-// its own, the buckets advancing together one bead per step (independent chains,
-// so their loads overlap; each a contiguous run of brec_sorted), then adds the
-// bucket sums in corner order.  A step per bead of the longest bucket, where the
-// bead-order merge takes one per bead of all eight.
+// its own (a contiguous run of brec_sorted), then adds the bucket sums in corner
+// order.  Eight lanes per voxel, lane k summing bucket k (the beads whose
+// lower-corner cell is v - (k & 1, k >> 1 & 1, k >> 2 & 1), wrapped; on grids one
+// or two voxels wide, where such cells coincide, the first corner of a cell takes
+// its bucket and folds every corner it stands for, in corner order).  The bucket
+// sums meet in LDS; lane m < B + 1 of the voxel adds species m's eight in corner
+// order and writes it.  A block holds GF_VOX voxels.
+constexpr int GF_VOX = 32;
+
 template <int NB>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(8 * GF_VOX)
 k_dens_gather_fixed(GridGeom G, const int32_t *off, const int32_t *cnt, const BeadRec *brec_sorted,
                     double *density)
 {
     constexpr int Bp1 = NB + 1;
-    const int64_t nbv = G.nx * G.ny * G.nz;
-    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (v >= nbv) return;
-    int32_t cell[8], pos[8], n[8];
-    int msk[8];
-    voxel_cells(G, v, cell, msk);
-    const double vvol = G.access_vol ? G.access_vol[v] : G.vol_bin;
-    int kmax = 0;
+    __shared__ double part[8 * GF_VOX][Bp1];
+    const int t = threadIdx.x, k = t & 7, j = t >> 3;
+    const int64_t nx = G.nx, ny = G.ny, nz = G.nz, nbv = nx * ny * nz;
+    const int64_t v = (int64_t)blockIdx.x * GF_VOX + j;
+    double acc[Bp1];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        pos[k] = msk[k] ? off[cell[k]] : 0;
-        n[k] = msk[k] ? cnt[cell[k]] : 0;
-        kmax = n[k] > kmax ? n[k] : kmax;
-    }
-    double acc[8][Bp1];
+    for (int m = 0; m < Bp1; ++m) acc[m] = 0;
+    double vvol = G.vol_bin;
+    if (v < nbv) {
+        const int64_t x = v % nx, y = (v / nx) % ny, z = v / (nx * ny);
+        auto cell_of = [&](int l) {
+            int64_t cx = x - (l & 1), cy = y - ((l >> 1) & 1), cz = z - ((l >> 2) & 1);
+            if (cx < 0) cx += nx;
+            if (cy < 0) cy += ny;
+            if (cz < 0) cz += nz;
+            return (int32_t)(cx + cy * nx + cz * nx * ny);
+        };
+        const int32_t ck = cell_of(k);
+        int msk = 0;
+        bool dup = false;
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
+        for (int l = 0; l < 8; ++l) {
+            if (cell_of(l) != ck) continue;
+            if (l < k) dup = true;
+            else msk |= 1 << l;
+        }
+        if (G.access_vol) vvol = G.access_vol[v];
+        const int32_t pos = off[ck], n = dup ? 0 : cnt[ck];
+        if (msk == 1 << k) {
+            for (int32_t i = 0; i < n; ++i) {
+                const BeadRec q = brec_sorted[pos + i];
+                const double d = corner_weight(k, q.w[0], q.w[1], q.w[2]) / vvol;
+                acc[0] += d;
 #pragma unroll
-        for (int m = 0; m < Bp1; ++m) acc[k][m] = 0;
-    for (int i = 0; i < kmax; ++i) {
+                for (int m = 1; m < Bp1; ++m) acc[m] += d * (double)q.st[m - 1];
+            }
+        } else {
+            for (int32_t i = 0; i < n; ++i) {
+                const BeadRec q = brec_sorted[pos + i];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            if (i >= n[k]) continue;
-            const BeadRec q = brec_sorted[pos[k] + i];
+                for (int l = 0; l < 8; ++l) {
+                    if (!(msk >> l & 1)) continue;
+                    const double d = corner_weight(l, q.w[0], q.w[1], q.w[2]) / vvol;
+                    acc[0] += d;
 #pragma unroll
-            for (int l = 0; l < 8; ++l) {
-                if (!(msk[k] >> l & 1)) continue;
-                const double d = corner_weight(l, q.w[0], q.w[1], q.w[2]) / vvol;
-                acc[k][0] += d;
-#pragma unroll
-                for (int m = 1; m < Bp1; ++m) acc[k][m] += d * (double)q.st[m - 1];
+                    for (int m = 1; m < Bp1; ++m) acc[m] += d * (double)q.st[m - 1];
+                }
             }
         }
     }
 #pragma unroll
-    for (int m = 0; m < Bp1; ++m) {
+    for (int m = 0; m < Bp1; ++m) part[t][m] = acc[m];
+    __syncthreads();
+    if (v < nbv && k < Bp1) {
         double a = 0;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) a += acc[k][m];
-        density[v * Bp1 + m] = a;
+        for (int c = 0; c < 8; ++c) a += part[8 * j + c][k];
+        density[v * Bp1 + k] = a;
     }
 }
 
@@ -475,12 +498,13 @@ inline hipError_t density_scatter(const GridGeom &G, int64_t N, int Bp1, PosView
     hipLaunchKernelGGL(k_dens_place, dim3(gb), dim3(256), 0, st, N, S.cell, S.off, S.fill, S.list);
     hipLaunchKernelGGL(k_dens_sort, dim3(gv), dim3(256), 0, st, nb, S.off, S.cnt, S.list);
     if (fixed_order) {
+        const unsigned gf = (unsigned)((nb + GF_VOX - 1) / GF_VOX);
         hipLaunchKernelGGL(k_dens_permute, dim3(gb), dim3(256), 0, st, N, S.list, S.brec, S.brec_sorted);
         switch (Bp1 - 1) {
-        case 1: hipLaunchKernelGGL(k_dens_gather_fixed<1>, dim3(gv), dim3(256), 0, st, G, S.off, S.cnt, S.brec_sorted, density); break;
-        case 2: hipLaunchKernelGGL(k_dens_gather_fixed<2>, dim3(gv), dim3(256), 0, st, G, S.off, S.cnt, S.brec_sorted, density); break;
-        case 3: hipLaunchKernelGGL(k_dens_gather_fixed<3>, dim3(gv), dim3(256), 0, st, G, S.off, S.cnt, S.brec_sorted, density); break;
-        default: hipLaunchKernelGGL(k_dens_gather_fixed<4>, dim3(gv), dim3(256), 0, st, G, S.off, S.cnt, S.brec_sorted, density); break;
+        case 1: hipLaunchKernelGGL(k_dens_gather_fixed<1>, dim3(gf), dim3(8 * GF_VOX), 0, st, G, S.off, S.cnt, S.brec_sorted, density); break;
+        case 2: hipLaunchKernelGGL(k_dens_gather_fixed<2>, dim3(gf), dim3(8 * GF_VOX), 0, st, G, S.off, S.cnt, S.brec_sorted, density); break;
+        case 3: hipLaunchKernelGGL(k_dens_gather_fixed<3>, dim3(gf), dim3(8 * GF_VOX), 0, st, G, S.off, S.cnt, S.brec_sorted, density); break;
+        default: hipLaunchKernelGGL(k_dens_gather_fixed<4>, dim3(gf), dim3(8 * GF_VOX), 0, st, G, S.off, S.cnt, S.brec_sorted, density); break;
         }
     } else if (nb >= (int64_t)1024 * GATHER_BIG_BLOCKS)
         hipLaunchKernelGGL((k_dens_gather<1024, true>), dim3((unsigned)((nb + 1023) / 1024)), dim3(1024), 0, st, G, Bp1,
