@@ -333,20 +333,22 @@ k_dens_gather_fixed(GridGeom G, const int32_t *off, const int32_t *cnt, const Be
     constexpr int Bp1 = NB + 1;
     __shared__ double part[8 * GF_VOX][Bp1];
     const int t = threadIdx.x, k = t & 7, j = t >> 3;
-    const int64_t nx = G.nx, ny = G.ny, nz = G.nz, nbv = nx * ny * nz;
-    const int64_t v = (int64_t)blockIdx.x * GF_VOX + j;
+    // 32-bit index arithmetic: the grids hold fewer than 2^31 voxels
+    const int32_t nx = (int32_t)G.nx, ny = (int32_t)G.ny, nz = (int32_t)G.nz, nbv = nx * ny * nz;
+    const int32_t v = (int32_t)blockIdx.x * GF_VOX + j;
     double acc[Bp1];
 #pragma unroll
     for (int m = 0; m < Bp1; ++m) acc[m] = 0;
     double vvol = G.vol_bin;
     if (v < nbv) {
-        const int64_t x = v % nx, y = (v / nx) % ny, z = v / (nx * ny);
+        const uint32_t q = (uint32_t)v / (uint32_t)nx;
+        const int32_t x = v - (int32_t)q * nx, y = (int32_t)(q % (uint32_t)ny), z = (int32_t)(q / (uint32_t)ny);
         auto cell_of = [&](int l) {
-            int64_t cx = x - (l & 1), cy = y - ((l >> 1) & 1), cz = z - ((l >> 2) & 1);
+            int32_t cx = x - (l & 1), cy = y - ((l >> 1) & 1), cz = z - ((l >> 2) & 1);
             if (cx < 0) cx += nx;
             if (cy < 0) cy += ny;
             if (cz < 0) cz += nz;
-            return (int32_t)(cx + cy * nx + cz * nx * ny);
+            return cx + nx * (cy + ny * cz);
         };
         const int32_t ck = cell_of(k);
         int msk = 0;
@@ -388,7 +390,7 @@ k_dens_gather_fixed(GridGeom G, const int32_t *off, const int32_t *cnt, const Be
         double a = 0;
 #pragma unroll
         for (int c = 0; c < 8; ++c) a += part[8 * j + c][k];
-        density[v * Bp1 + k] = a;
+        density[(int64_t)v * Bp1 + k] = a;
     }
 }
 

@@ -240,7 +240,7 @@ __device__ inline void E_pair_grad_dr(const Bond &bd, const double *r0, const do
 // cell and brec: each bead's lower-corner voxel and weights from the density
 // scatter of the same positions (grid_lower's values, not recomputed).
 template <int B>
-__global__ void __launch_bounds__(EB) k_e_bead(EModel M, const double *theta, const double *r,
+__global__ void __launch_bounds__(EB) __attribute__((amdgpu_waves_per_eu(4))) k_e_bead(EModel M, const double *theta, const double *r,
                                                const double *dEdrho, const int32_t *cell, const BeadRec *brec,
                                                double *grad, double *part)
 {
@@ -251,7 +251,12 @@ __global__ void __launch_bounds__(EB) k_e_bead(EModel M, const double *theta, co
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = 0;
     const double f = M.mu_adjust;
+    // +-1 / dx is exactly +-(1 / dx)
+    const double inv_d[3] = {1.0 / M.G.dxyz[0], 1.0 / M.G.dxyz[1], 1.0 / M.G.dxyz[2]};
     for (int64_t b = (int64_t)blockIdx.x * EB + threadIdx.x; b < N; b += (int64_t)gridDim.x * EB) {
+        int st[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) st[j] = M.states[(int64_t)j * N + b];
         const double *rb = r + 3 * b;
         const double *tb = M.t3 + 3 * b;
         if (b + 1 < N) {
@@ -260,7 +265,7 @@ __global__ void __launch_bounds__(EB) k_e_bead(EModel M, const double *theta, co
         }
 #pragma unroll
         for (int j = 0; j < B; ++j) {
-            int s = M.states[(int64_t)j * N + b];
+            const int s = st[j];
             int m = M.marks[(int64_t)j * N + b];
             double mu = c.mu[j];
             double e = M.F[j][m][s];
@@ -272,24 +277,27 @@ __global__ void __launch_bounds__(EB) k_e_bead(EModel M, const double *theta, co
         if (grad) {
             double g[3] = {0, 0, 0};
             if (dEdrho) {
-                const int64_t c = cell[b];
-                const int64_t ix[3] = {c % M.G.nx, (c / M.G.nx) % M.G.ny, c / (M.G.nx * M.G.ny)};
+                // 32-bit index arithmetic: the grids hold fewer than 2^31 voxels
+                const int32_t nx = (int32_t)M.G.nx, ny = (int32_t)M.G.ny, nz = (int32_t)M.G.nz;
+                const uint32_t c = (uint32_t)cell[b], cq = c / (uint32_t)nx;
+                const int32_t ix[3] = {(int32_t)(c - cq * (uint32_t)nx), (int32_t)(cq % (uint32_t)ny),
+                                       (int32_t)(cq / (uint32_t)ny)};
                 const double w[3] = {brec[b].w[0], brec[b].w[1], brec[b].w[2]};
                 for (int l = 0; l < 8; ++l) {
-                    int64_t x = ix[0] + (l & 1), y = ix[1] + ((l >> 1) & 1), z = ix[2] + ((l >> 2) & 1);
-                    if (x >= M.G.nx) x -= M.G.nx;
-                    if (y >= M.G.ny) y -= M.G.ny;
-                    if (z >= M.G.nz) z -= M.G.nz;
-                    int64_t v = x + y * M.G.nx + z * M.G.nx * M.G.ny;
+                    int32_t x = ix[0] + (l & 1), y = ix[1] + ((l >> 1) & 1), z = ix[2] + ((l >> 2) & 1);
+                    if (x >= nx) x -= nx;
+                    if (y >= ny) y -= ny;
+                    if (z >= nz) z -= nz;
+                    const int64_t v = x + nx * (y + ny * z);
                     const double *d = dEdrho + v * Bp1;
                     double h = d[0];
 #pragma unroll
-                    for (int j = 0; j < B; ++j) h += (double)M.states[(int64_t)j * N + b] * d[j + 1];
+                    for (int j = 0; j < B; ++j) h += (double)st[j] * d[j + 1];
                     h /= M.G.vol_bin;
                     double a[3];
                     for (int j = 0; j < 3; ++j) a[j] = ((l >> j) & 1) ? (1 - w[j]) : w[j];
                     for (int j = 0; j < 3; ++j) {
-                        double dw = (((l >> j) & 1) ? 1.0 : -1.0) / M.G.dxyz[j];
+                        double dw = ((l >> j) & 1) ? inv_d[j] : -inv_d[j];
                         double prod = dw;
                         for (int q = 0; q < 3; ++q)
                             if (q != j) prod *= a[q];
@@ -521,10 +529,12 @@ k_ref_final(EModel M, const double *pref, const double *intra, const double *vpa
     out[1] = E;
 }
 
+// Blocks of a voxel or bead kernel: at most 2048 (eight waves per SIMD of the
+// device), so that the final sums read few partials per term.
 int grid_blocks(int64_t n)
 {
     int64_t g = (n + EB - 1) / EB;
-    if (g > 8192) g = 8192;
+    if (g > 2048) g = 2048;
     if (g < 1) g = 1;
     return (int)g;
 }
