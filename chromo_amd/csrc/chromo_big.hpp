@@ -123,68 +123,58 @@ __device__ __forceinline__ void big_number_coop(const BigLds &S, int rounds, int
                                                 int wv, int &T, int &U)
 {
     constexpr int Bp1 = NB + 1;
-    constexpr int K = (BIG_ROUNDS + 1) / 2;
     const int lane = lane_id();
-    bool first[K], fresh[K];
-    int hs[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int r = wv + 2 * k;
-        const int e = lane + WAVE * r;
-        first[k] = false;
-        fresh[k] = false;
-        hs[k] = 0;
-        if (r < rounds && e < E) {
-            const int p = e >> 3, l = e & 7;
-            const int h = S.eslot[e];
-            bool f = (S.hpos[h] & ((1ull << p) - 1)) == 0;   // no earlier position touches it
-            if (f && degen) {
-                // corners of one position can coincide on a one-voxel axis
-                const int4 pr = S.prec[p];
-                const int v = big_corner(pr, l, nx, ny);
-                for (int l2 = 0; l2 < l; ++l2)
-                    if (big_corner(pr, l2, nx, ny) == v) f = false;
-            }
-            first[k] = f;
-            fresh[k] = f && S.hq[h] < 0;
-            hs[k] = h;
+    // contribution e's slot, whether it is the slot's first in the chunk, and
+    // whether that voxel is new; recomputed in pass 2 rather than held across
+    // the barrier (hq[h] is written only by h's first contribution, after its read)
+    auto classify = [&](int e, int &h, bool &first, bool &fresh) {
+        first = false;
+        fresh = false;
+        h = 0;
+        if (e >= E) return;
+        const int p = e >> 3, l = e & 7;
+        h = S.eslot[e];
+        bool f = (S.hpos[h] & ((1ull << p) - 1)) == 0;   // no earlier position touches it
+        if (f && degen) {
+            // corners of one position can coincide on a one-voxel axis
+            const int4 pr = S.prec[p];
+            const int v = big_corner(pr, l, nx, ny);
+            for (int l2 = 0; l2 < l; ++l2)
+                if (big_corner(pr, l2, nx, ny) == v) f = false;
         }
-        if (r < rounds) {
-            const int ct = __popcll(__ballot(first[k])), cn = __popcll(__ballot(fresh[k]));
-            if (lane == 0) {
-                S.xch[r] = ct;
-                S.xch[BIG_ROUNDS + r] = cn;
-            }
+        first = f;
+        fresh = f && S.hq[h] < 0;
+    };
+    for (int r = wv; r < rounds; r += 2) {
+        int h;
+        bool first, fresh;
+        classify(lane + WAVE * r, h, first, fresh);
+        const int ct = __popcll(__ballot(first)), cn = __popcll(__ballot(fresh));
+        if (lane == 0) {
+            S.xch[r] = ct;
+            S.xch[BIG_ROUNDS + r] = cn;
         }
     }
     __syncthreads();
-    int tk[K], uk[K];
     int toff = T, uoff = U;
+    for (int r = 0; r < rounds; ++r) {
+        if ((r & 1) == wv) {
+            int h;
+            bool first, fresh;
+            classify(lane + WAVE * r, h, first, fresh);
+            int tcnt, ncnt;
+            const int t = toff + wave_excl_count(first, tcnt);
+            if (first) S.tlist[t] = h;
+            const int q = uoff + wave_excl_count(fresh, ncnt);
+            if (fresh && q < S.ucap) {
+                S.hq[h] = q;
+                S.uvox[q] = S.hkey[h];
 #pragma unroll
-    for (int r = 0; r < BIG_ROUNDS; ++r) {
-        if (r < rounds) {
-            if ((r & 1) == wv) {
-                tk[r >> 1] = toff;
-                uk[r >> 1] = uoff;
+                for (int m = 0; m < Bp1; ++m) S.uacc[q * Bp1 + m] = 0;
             }
-            toff += S.xch[r];
-            uoff += S.xch[BIG_ROUNDS + r];
         }
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int r = wv + 2 * k;
-        if (r >= rounds) continue;
-        int tcnt, ncnt;
-        const int t = tk[k] + wave_excl_count(first[k], tcnt);
-        if (first[k]) S.tlist[t] = hs[k];
-        const int q = uk[k] + wave_excl_count(fresh[k], ncnt);
-        if (fresh[k] && q < S.ucap) {
-            S.hq[hs[k]] = q;
-            S.uvox[q] = S.hkey[hs[k]];
-#pragma unroll
-            for (int m = 0; m < Bp1; ++m) S.uacc[q * Bp1 + m] = 0;
-        }
+        toff += S.xch[r];
+        uoff += S.xch[BIG_ROUNDS + r];
     }
     T = toff;
     U = uoff;
