@@ -352,8 +352,9 @@ def test_twist_absolute_energy(oracle):
 def test_long_window_blocks_two_wavefronts(oracle, rng_mode):
     """Steady-state amplitudes (crank-shaft, slide and end-pivot windows of 100-150
     beads) with two wavefronts: about a third of those proposals exceed the
-    speculative evaluator and run as sequential slots (the 64-bead and chunked
-    evaluators) between speculative rounds; the chain is the oracle's."""
+    speculative evaluator and run as sequential slots between speculative rounds,
+    evaluated by both wavefronts (the 64-bead and chunked evaluators); the chain
+    is the oracle's."""
     say("long windows, W=2", rng_mode)
     s = W.make_system(6000, 2, 24, seed=97, eps=[-4, -2], mu=[-0.4, 0.3])
     for c in s.ctrls:
@@ -361,3 +362,17 @@ def test_long_window_blocks_two_wavefronts(oracle, rng_mode):
             c.amp_bead, c.bead_lo, c.bead_hi = 130, 100.0, 150.0
     lo, _ = _run_pair(oracle, s, 6, rng_mode=rng_mode, waves=2, rel=1e-9)
     assert np.any(lo["n"] > 64)
+
+
+@pytest.mark.parametrize("B", [1, 3, 4])
+def test_long_window_blocks_two_wavefronts_binder_counts(oracle, B):
+    """The two-wavefront sequential evaluation (33-64-bead and chunked windows)
+    in the launches of the other binder counts, against the oracle."""
+    say("long windows, W=2, B =", B)
+    s = W.make_system(2500, B, 16, seed=60 + B, eps=-3.0, cross_eps=0.3 if B > 1 else 0.0, mu=-0.5)
+    for c in s.ctrls:
+        if c.move in (L.CRANK_SHAFT, L.SLIDE, L.END_PIVOT):
+            c.amp_bead, c.bead_lo, c.bead_hi = 120, 40.0, 150.0
+    lo, _ = _run_pair(oracle, s, 5, rng_mode=L.RNG_PHILOX, waves=2, rel=1e-9)
+    n = lo["n"]
+    assert np.any((n > 32) & (n <= 64)) and np.any(n > 64)
