@@ -205,8 +205,10 @@ __global__ void __launch_bounds__(EB) k_e_voxel(EModel M, const double *theta, c
 }
 
 // ---- bead / bond terms -------------------------------------------------------
-__device__ inline double E_pair_abs(const Bond &bd, const double *r0, const double *r1,
-                                    const double *t30, const double *t31)
+// Elastic energy of bond (r0, t30) -> (r1, t31), and (grad) its derivative with
+// respect to r1 - r0 with the tangents fixed, from the same intermediates.
+__device__ inline double E_pair_and_grad(const Bond &bd, const double *r0, const double *r1, const double *t30,
+                                         const double *t31, bool grad, double g[3])
 {
     double dr[3], drp[3], be[3];
     for (int i = 0; i < 3; ++i) dr[i] = r1[i] - r0[i];
@@ -214,33 +216,22 @@ __device__ inline double E_pair_abs(const Bond &bd, const double *r0, const doub
     for (int i = 0; i < 3; ++i) {
         drp[i] = dr[i] - t30[i] * par;
         be[i] = t31[i] - t30[i] - drp[i] * bd.eta;
+    }
+    if (grad) {
+        // perp = P dr with P = I - t t^T; bend = t31 - t30 - eta perp
+        double a[3];
+        for (int i = 0; i < 3; ++i) a[i] = bd.eperp * drp[i] - bd.eta * bd.eb * be[i];
+        double ta = dot3(t30, a);
+        for (int i = 0; i < 3; ++i) g[i] = (a[i] - t30[i] * ta) + bd.ep * (par - bd.gam) * t30[i];
     }
     return E_pair(bd, be, par, drp);
-}
-
-// dE_pair / d(r1 - r0) with tangents fixed
-__device__ inline void E_pair_grad_dr(const Bond &bd, const double *r0, const double *r1,
-                                      const double *t30, const double *t31, double g[3])
-{
-    double dr[3], drp[3], be[3];
-    for (int i = 0; i < 3; ++i) dr[i] = r1[i] - r0[i];
-    double par = dot3(t30, dr);
-    for (int i = 0; i < 3; ++i) {
-        drp[i] = dr[i] - t30[i] * par;
-        be[i] = t31[i] - t30[i] - drp[i] * bd.eta;
-    }
-    // perp = P dr with P = I - t t^T; bend = t31 - t30 - eta perp
-    double a[3];
-    for (int i = 0; i < 3; ++i) a[i] = bd.eperp * drp[i] - bd.eta * bd.eb * be[i];
-    double ta = dot3(t30, a);
-    for (int i = 0; i < 3; ++i) g[i] = (a[i] - t30[i] * ta) + bd.ep * (par - bd.gam) * t30[i];
 }
 
 // slots: [0] E_elastic, [1] E_bind, [2..2+B) D_j, [2+B..2+2B) sum s_j
 // cell and brec: each bead's lower-corner voxel and weights from the density
 // scatter of the same positions (grid_lower's values, not recomputed).
 template <int B>
-__global__ void __launch_bounds__(EB) __attribute__((amdgpu_waves_per_eu(4))) k_e_bead(EModel M, const double *theta, const double *r,
+__global__ void __launch_bounds__(EB) k_e_bead(EModel M, const double *theta, const double *r,
                                                const double *dEdrho, const int32_t *cell, const BeadRec *brec,
                                                double *grad, double *part)
 {
@@ -253,30 +244,45 @@ __global__ void __launch_bounds__(EB) __attribute__((amdgpu_waves_per_eu(4))) k_
     const double f = M.mu_adjust;
     // +-1 / dx is exactly +-(1 / dx)
     const double inv_d[3] = {1.0 / M.G.dxyz[0], 1.0 / M.G.dxyz[1], 1.0 / M.G.dxyz[2]};
-    for (int64_t b = (int64_t)blockIdx.x * EB + threadIdx.x; b < N; b += (int64_t)gridDim.x * EB) {
-        int st[B];
-#pragma unroll
-        for (int j = 0; j < B; ++j) st[j] = M.states[(int64_t)j * N + b];
+    // bond i = (i, i + 1): each is evaluated once, by thread i - b0 of the block
+    // that holds bead i (the block's first bond (b0 - 1, b0) by its last thread);
+    // the derivatives meet in LDS, where beads b and b + 1 read them
+    __shared__ double gbond[EB + 1][3];
+    const int t = threadIdx.x;
+    for (int64_t b0 = (int64_t)blockIdx.x * EB; b0 < N; b0 += (int64_t)gridDim.x * EB) {
+        const int64_t b = b0 + t;
         const double *rb = r + 3 * b;
         const double *tb = M.t3 + 3 * b;
         if (b + 1 < N) {
             Bond bd = load_bond(M.bonds, N, b);
-            acc[0] += E_pair_abs(bd, rb, r + 3 * (b + 1), tb, M.t3 + 3 * (b + 1));
+            double gd[3];
+            acc[0] += E_pair_and_grad(bd, rb, r + 3 * (b + 1), tb, M.t3 + 3 * (b + 1), grad != nullptr, gd);
+            if (grad)
+                for (int j = 0; j < 3; ++j) gbond[t + 1][j] = gd[j];
         }
+        if (grad && t == EB - 1 && b0 > 0) {
+            Bond bd = load_bond(M.bonds, N, b0 - 1);
+            double gd[3];
+            E_pair_and_grad(bd, r + 3 * (b0 - 1), r + 3 * b0, M.t3 + 3 * (b0 - 1), M.t3 + 3 * b0, true, gd);
+            for (int j = 0; j < 3; ++j) gbond[0][j] = gd[j];
+        }
+        double g[3] = {0, 0, 0};
+        if (b < N) {
+            int st[B];
 #pragma unroll
-        for (int j = 0; j < B; ++j) {
-            const int s = st[j];
-            int m = M.marks[(int64_t)j * N + b];
-            double mu = c.mu[j];
-            double e = M.F[j][m][s];
-            e += (mu > 0) ? -(double)s * (mu * (2 - f)) : -(double)s * mu * f;
-            acc[1] += e;
-            acc[2 + j] += (s == 2);
-            acc[2 + B + j] += (double)s * ((mu > 0) ? (2 - f) : f);
-        }
-        if (grad) {
-            double g[3] = {0, 0, 0};
-            if (dEdrho) {
+            for (int j = 0; j < B; ++j) st[j] = M.states[(int64_t)j * N + b];
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                const int s = st[j];
+                int m = M.marks[(int64_t)j * N + b];
+                double mu = c.mu[j];
+                double e = M.F[j][m][s];
+                e += (mu > 0) ? -(double)s * (mu * (2 - f)) : -(double)s * mu * f;
+                acc[1] += e;
+                acc[2 + j] += (s == 2);
+                acc[2 + B + j] += (double)s * ((mu > 0) ? (2 - f) : f);
+            }
+            if (grad && dEdrho) {
                 // 32-bit index arithmetic: the grids hold fewer than 2^31 voxels
                 const int32_t nx = (int32_t)M.G.nx, ny = (int32_t)M.G.ny, nz = (int32_t)M.G.nz;
                 const uint32_t c = (uint32_t)cell[b], cq = c / (uint32_t)nx;
@@ -305,19 +311,18 @@ __global__ void __launch_bounds__(EB) __attribute__((amdgpu_waves_per_eu(4))) k_
                     }
                 }
             }
-            if (b + 1 < N) {   // bond (b, b+1): r0 = r_b
-                Bond bd = load_bond(M.bonds, N, b);
-                double gd[3];
-                E_pair_grad_dr(bd, rb, r + 3 * (b + 1), tb, M.t3 + 3 * (b + 1), gd);
-                for (int j = 0; j < 3; ++j) g[j] -= gd[j];
+        }
+        if (grad) {
+            // every thread of the block reaches both barriers
+            __syncthreads();
+            if (b < N) {
+                if (b + 1 < N)   // bond (b, b+1): r0 = r_b
+                    for (int j = 0; j < 3; ++j) g[j] -= gbond[t + 1][j];
+                if (b > 0)       // bond (b-1, b): r1 = r_b
+                    for (int j = 0; j < 3; ++j) g[j] += gbond[t][j];
+                for (int j = 0; j < 3; ++j) grad[3 * b + j] = g[j];
             }
-            if (b > 0) {       // bond (b-1, b): r1 = r_b
-                Bond bd = load_bond(M.bonds, N, b - 1);
-                double gd[3];
-                E_pair_grad_dr(bd, r + 3 * (b - 1), rb, M.t3 + 3 * (b - 1), tb, gd);
-                for (int j = 0; j < 3; ++j) g[j] += gd[j];
-            }
-            for (int j = 0; j < 3; ++j) grad[3 * b + j] = g[j];
+            __syncthreads();
         }
     }
     block_reduce_store<NT>(acc, part);
