@@ -306,40 +306,62 @@ k_dens_gather(GridGeom G, int Bp1, const int32_t *off, const int32_t *cnt, const
     }
 }
 
-// The bead records in bucket order (brec_sorted[i] = brec[list[i]]): a bucket's
-// beads are then one contiguous run.
-__global__ void k_dens_permute(int64_t N, const int32_t *list, const BeadRec *brec, BeadRec *brec_sorted)
+// The energy adjoint's placement: bead b's record goes straight into its cell's
+// bucket, in the order the atomic counter hands out (the fixed-point gather below
+// does not depend on it).
+__global__ void k_dens_place_rec(int64_t N, const int32_t *cell, const int32_t *off, int32_t *fill,
+                                 const BeadRec *brec, BeadRec *brec_bucketed)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N) brec_sorted[i] = brec[list[i]];
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= N) return;
+    const int32_t c = cell[b];
+    brec_bucketed[off[c] + atomicAdd(&fill[c], 1)] = brec[b];
 }
 
 // The density of the energy adjoint, which needs a deterministic sum but not the
-// reference's bead-order fold: voxel v sums each of its (<= 8) sorted buckets on
-// its own (a contiguous run of brec_sorted), then adds the bucket sums in corner
-// order.  Eight lanes per voxel, lane k summing bucket k (the beads whose
-// lower-corner cell is v - (k & 1, k >> 1 & 1, k >> 2 & 1), wrapped; on grids one
-// or two voxels wide, where such cells coincide, the first corner of a cell takes
-// its bucket and folds every corner it stands for, in corner order).  The bucket
-// sums meet in LDS; lane m < B + 1 of the voxel adds species m's eight in corner
-// order and writes it.  A block holds GF_VOX voxels.
+// reference's bead-order fold.  Each corner weight w (in [0, 1]) is turned into
+// the integer floor(w 2^60) (within 2^-60 of w) and the integers are summed in
+// 128 bits, times the binder state (0..CHROMO_MAX_SITES) for the binder rows: the
+// sums are exact, so they do not depend on the order of the beads in a bucket,
+// and the atomic placement needs no sort.  The voxel's total is rounded to double
+// once and divided by its volume.  Eight lanes per voxel, lane k summing bucket k
+// (the beads whose lower-corner cell is v - (k & 1, k >> 1 & 1, k >> 2 & 1),
+// wrapped; on grids one or two voxels wide, where such cells coincide, the first
+// corner of a cell takes its bucket and adds every corner it stands for).  The
+// bucket sums meet in LDS; lane m < B + 1 of the voxel adds species m's eight and
+// writes it.  A block holds GF_VOX voxels.  The sums stay below 2^96 (2^31 beads,
+// eight corners, states up to 3).
 constexpr int GF_VOX = 32;
+
+__device__ __forceinline__ uint64_t fixed60(double w) { return (uint64_t)(w * 0x1p60); }
+
+__device__ __forceinline__ double from_fixed60(unsigned __int128 a)
+{
+    return ((double)(uint64_t)(a >> 64) * 0x1p64 + (double)(uint64_t)a) * 0x1p-60;
+}
+
+template <int NB>
+__device__ __forceinline__ void add_fixed(unsigned __int128 (&acc)[NB + 1], uint64_t w, const int8_t *st)
+{
+    acc[0] += w;
+#pragma unroll
+    for (int m = 1; m <= NB; ++m) acc[m] += w * (uint64_t)(uint8_t)st[m - 1];
+}
 
 template <int NB>
 __global__ void __launch_bounds__(8 * GF_VOX)
-k_dens_gather_fixed(GridGeom G, const int32_t *off, const int32_t *cnt, const BeadRec *brec_sorted,
+k_dens_gather_fixed(GridGeom G, const int32_t *off, const int32_t *cnt, const BeadRec *brec_bucketed,
                     double *density)
 {
     constexpr int Bp1 = NB + 1;
-    __shared__ double part[8 * GF_VOX][Bp1];
+    __shared__ unsigned __int128 part[8 * GF_VOX][Bp1];
     const int t = threadIdx.x, k = t & 7, j = t >> 3;
     // 32-bit index arithmetic: the grids hold fewer than 2^31 voxels
     const int32_t nx = (int32_t)G.nx, ny = (int32_t)G.ny, nz = (int32_t)G.nz, nbv = nx * ny * nz;
     const int32_t v = (int32_t)blockIdx.x * GF_VOX + j;
-    double acc[Bp1];
+    unsigned __int128 acc[Bp1];
 #pragma unroll
     for (int m = 0; m < Bp1; ++m) acc[m] = 0;
-    double vvol = G.vol_bin;
     if (v < nbv) {
         const uint32_t q = (uint32_t)v / (uint32_t)nx;
         const int32_t x = v - (int32_t)q * nx, y = (int32_t)(q % (uint32_t)ny), z = (int32_t)(q / (uint32_t)ny);
@@ -359,27 +381,17 @@ k_dens_gather_fixed(GridGeom G, const int32_t *off, const int32_t *cnt, const Be
             if (l < k) dup = true;
             else msk |= 1 << l;
         }
-        if (G.access_vol) vvol = G.access_vol[v];
         const int32_t pos = off[ck], n = dup ? 0 : cnt[ck];
         if (msk == 1 << k) {
             for (int32_t i = 0; i < n; ++i) {
-                const BeadRec q = brec_sorted[pos + i];
-                const double d = corner_weight(k, q.w[0], q.w[1], q.w[2]) / vvol;
-                acc[0] += d;
-#pragma unroll
-                for (int m = 1; m < Bp1; ++m) acc[m] += d * (double)q.st[m - 1];
+                const BeadRec r = brec_bucketed[pos + i];
+                add_fixed<NB>(acc, fixed60(corner_weight(k, r.w[0], r.w[1], r.w[2])), r.st);
             }
         } else {
             for (int32_t i = 0; i < n; ++i) {
-                const BeadRec q = brec_sorted[pos + i];
-#pragma unroll
-                for (int l = 0; l < 8; ++l) {
-                    if (!(msk >> l & 1)) continue;
-                    const double d = corner_weight(l, q.w[0], q.w[1], q.w[2]) / vvol;
-                    acc[0] += d;
-#pragma unroll
-                    for (int m = 1; m < Bp1; ++m) acc[m] += d * (double)q.st[m - 1];
-                }
+                const BeadRec r = brec_bucketed[pos + i];
+                for (int l = 0; l < 8; ++l)
+                    if (msk >> l & 1) add_fixed<NB>(acc, fixed60(corner_weight(l, r.w[0], r.w[1], r.w[2])), r.st);
             }
         }
     }
@@ -387,10 +399,11 @@ k_dens_gather_fixed(GridGeom G, const int32_t *off, const int32_t *cnt, const Be
     for (int m = 0; m < Bp1; ++m) part[t][m] = acc[m];
     __syncthreads();
     if (v < nbv && k < Bp1) {
-        double a = 0;
+        unsigned __int128 a = 0;
 #pragma unroll
         for (int c = 0; c < 8; ++c) a += part[8 * j + c][k];
-        density[(int64_t)v * Bp1 + k] = a;
+        const double vvol = G.access_vol ? G.access_vol[v] : G.vol_bin;
+        density[(int64_t)v * Bp1 + k] = from_fixed60(a) / vvol;
     }
 }
 
@@ -398,7 +411,7 @@ k_dens_gather_fixed(GridGeom G, const int32_t *off, const int32_t *cnt, const Be
 struct DensityScratch {
     int32_t *cell = nullptr, *list = nullptr, *cnt = nullptr, *off = nullptr, *fill = nullptr;
     int32_t *bsum = nullptr, *boff = nullptr, *bsum2 = nullptr;
-    BeadRec *brec = nullptr, *brec_sorted = nullptr;
+    BeadRec *brec = nullptr, *brec_bucketed = nullptr;
     int64_t maxN = 0, maxBins = 0;
 };
 
@@ -407,10 +420,10 @@ constexpr size_t SCRATCH_ALIGN = 256;
 inline size_t align_scratch(size_t o) { return (o + SCRATCH_ALIGN - 1) / SCRATCH_ALIGN * SCRATCH_ALIGN; }
 
 // The scatter's scratch carved from one buffer at base (nullptr: sizes only);
-// returns the bytes used, a multiple of SCRATCH_ALIGN.  sorted_records: room for
-// the bucket-ordered bead records of the fixed-order gather.
+// returns the bytes used, a multiple of SCRATCH_ALIGN.  bucketed_records: room for
+// the bead records placed by bucket for the fixed-point gather.
 inline size_t density_scratch_carve(DensityScratch &S, char *base, int64_t maxN, int64_t maxBins,
-                                    bool sorted_records)
+                                    bool bucketed_records)
 {
     size_t o = 0;
     auto take = [&](size_t bytes) {
@@ -423,7 +436,7 @@ inline size_t density_scratch_carve(DensityScratch &S, char *base, int64_t maxN,
     S.cell = (int32_t *)take(sizeof(int32_t) * maxN);
     S.list = (int32_t *)take(sizeof(int32_t) * maxN);
     S.brec = (BeadRec *)take(sizeof(BeadRec) * maxN);
-    S.brec_sorted = sorted_records ? (BeadRec *)take(sizeof(BeadRec) * maxN) : nullptr;
+    S.brec_bucketed = bucketed_records ? (BeadRec *)take(sizeof(BeadRec) * maxN) : nullptr;
     S.cnt = (int32_t *)take(sizeof(int32_t) * maxBins);
     S.off = (int32_t *)take(sizeof(int32_t) * maxBins);
     S.fill = (int32_t *)take(sizeof(int32_t) * maxBins);
@@ -460,7 +473,7 @@ inline void density_scratch_free(DensityScratch &S, hipStream_t st)
     for (auto p : ip)
         if (p) (void)hipFreeAsync(p, st);
     if (S.brec) (void)hipFreeAsync(S.brec, st);
-    if (S.brec_sorted) (void)hipFreeAsync(S.brec_sorted, st);
+    if (S.brec_bucketed) (void)hipFreeAsync(S.brec_bucketed, st);
     S = DensityScratch();
 }
 
@@ -475,10 +488,10 @@ __global__ void k_fill2_i32(int32_t *p, int32_t *q, int64_t n)
 }
 
 // Scatter N beads into density (n_bins x Bp1).  Requires N <= S.maxN,
-// n_bins <= S.maxBins <= 2048 * 2048.  fixed_order: the adjoint's deterministic
-// corner-by-corner sum (k_dens_gather_fixed; S carved with sorted_records) instead
-// of the reference's bead-order fold; cleanup (the 1e-18 rule of the initial
-// densities) applies to the fold only.
+// n_bins <= S.maxBins <= 2048 * 2048.  fixed_order: the adjoint's exact
+// fixed-point sum over unsorted buckets (k_dens_place_rec, k_dens_gather_fixed; S
+// carved with bucketed_records) instead of the reference's bead-order fold; cleanup
+// (the 1e-18 rule of the initial densities) applies to the fold only.
 inline hipError_t density_scatter(const GridGeom &G, int64_t N, int Bp1, PosView P,
                                   const int8_t *states, double *density, int cleanup,
                                   DensityScratch &S, hipStream_t st, bool fixed_order = false)
@@ -497,18 +510,21 @@ inline hipError_t density_scatter(const GridGeom &G, int64_t N, int Bp1, PosView
     hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nbl), dim3(1024), 0, st, S.cnt, S.off, S.bsum, nb);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, S.bsum, S.boff, S.bsum2, nbl);
     hipLaunchKernelGGL(k_scan_add, dim3(gv), dim3(256), 0, st, S.off, S.boff, nb);
-    hipLaunchKernelGGL(k_dens_place, dim3(gb), dim3(256), 0, st, N, S.cell, S.off, S.fill, S.list);
-    hipLaunchKernelGGL(k_dens_sort, dim3(gv), dim3(256), 0, st, nb, S.off, S.cnt, S.list);
     if (fixed_order) {
         const unsigned gf = (unsigned)((nb + GF_VOX - 1) / GF_VOX);
-        hipLaunchKernelGGL(k_dens_permute, dim3(gb), dim3(256), 0, st, N, S.list, S.brec, S.brec_sorted);
+        hipLaunchKernelGGL(k_dens_place_rec, dim3(gb), dim3(256), 0, st, N, S.cell, S.off, S.fill, S.brec,
+                           S.brec_bucketed);
         switch (Bp1 - 1) {
-        case 1: hipLaunchKernelGGL(k_dens_gather_fixed<1>, dim3(gf), dim3(8 * GF_VOX), 0, st, G, S.off, S.cnt, S.brec_sorted, density); break;
-        case 2: hipLaunchKernelGGL(k_dens_gather_fixed<2>, dim3(gf), dim3(8 * GF_VOX), 0, st, G, S.off, S.cnt, S.brec_sorted, density); break;
-        case 3: hipLaunchKernelGGL(k_dens_gather_fixed<3>, dim3(gf), dim3(8 * GF_VOX), 0, st, G, S.off, S.cnt, S.brec_sorted, density); break;
-        default: hipLaunchKernelGGL(k_dens_gather_fixed<4>, dim3(gf), dim3(8 * GF_VOX), 0, st, G, S.off, S.cnt, S.brec_sorted, density); break;
+        case 1: hipLaunchKernelGGL(k_dens_gather_fixed<1>, dim3(gf), dim3(8 * GF_VOX), 0, st, G, S.off, S.cnt, S.brec_bucketed, density); break;
+        case 2: hipLaunchKernelGGL(k_dens_gather_fixed<2>, dim3(gf), dim3(8 * GF_VOX), 0, st, G, S.off, S.cnt, S.brec_bucketed, density); break;
+        case 3: hipLaunchKernelGGL(k_dens_gather_fixed<3>, dim3(gf), dim3(8 * GF_VOX), 0, st, G, S.off, S.cnt, S.brec_bucketed, density); break;
+        default: hipLaunchKernelGGL(k_dens_gather_fixed<4>, dim3(gf), dim3(8 * GF_VOX), 0, st, G, S.off, S.cnt, S.brec_bucketed, density); break;
         }
-    } else if (nb >= (int64_t)1024 * GATHER_BIG_BLOCKS)
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_dens_place, dim3(gb), dim3(256), 0, st, N, S.cell, S.off, S.fill, S.list);
+    hipLaunchKernelGGL(k_dens_sort, dim3(gv), dim3(256), 0, st, nb, S.off, S.cnt, S.list);
+    if (nb >= (int64_t)1024 * GATHER_BIG_BLOCKS)
         hipLaunchKernelGGL((k_dens_gather<1024, true>), dim3((unsigned)((nb + 1023) / 1024)), dim3(1024), 0, st, G, Bp1,
                            S.off, S.cnt, S.list, S.brec, density, cleanup);
     else
