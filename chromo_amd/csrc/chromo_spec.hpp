@@ -903,8 +903,8 @@ __device__ __forceinline__ void spec_block(const chromo_replica *R, WaveRng &rng
                 }
                 wave_sync();
             }
-            // wavefront 0 may use all W evaluator blocks as one wide block; a
-            // 64-bead move both evaluate
+            // the W evaluator blocks serve as one wide block; both wavefronts
+            // enter, and evaluate a long crank-shaft, slide or end-pivot together
             seq_step_coop<NB, W>(R, L.seq, L.mt, L.first, L.wide);
             if (wv == 0) {
                 SeqIO *io = L.seq;
